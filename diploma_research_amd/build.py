@@ -1,0 +1,116 @@
+"""Build recipe for libdim_gpu.so (gfx950) and the dim CLI.
+
+    python -m diploma_research_amd.build          # library + CLI
+    python -m diploma_research_amd.build --lib    # library only
+
+Every HIP/C++ source is compiled by hipcc for gfx950 in parallel into
+_build/, then linked in-tree as diploma_research_amd/libdim_gpu.so.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "libdim_gpu.so")
+CLI = os.path.join(PKG, "dim")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+SOURCES = ["format.cpp", "capi.cpp", "spmv.hip", "spmv_k2.hip", "spmv_k4.hip", "spmv_k8.hip", "spmv_k16.hip",
+           "cg.hip", "convert.hip"]
+
+# Packed FP32 instructions are switched off for all device code (some of their
+# forms have given wrong results on these machines).
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-Wall", "-Wno-unused-command-line-argument", "-I", CSRC,
+             "-I", os.path.join(ROOT, "include")]
+
+# HDF5 (C API) for the CLI: the image ships it under /opt/conda.
+HDF5_ROOT = os.environ.get("DIM_HDF5_ROOT", "/opt/conda")
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l and "target-feature" not in l)
+    return p.returncode, out
+
+
+def _compile(src):
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    path = os.path.join(CSRC, src)
+    deps = [path] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    deps.append(os.path.join(ROOT, "include", "dim_gpu.h"))
+    if os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
+        return src, 0, "(up to date)"
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    rc, out = _run([HIPCC, *HIP_FLAGS, *lang, "-c", path, "-o", obj])
+    return src, rc, out
+
+
+def build_lib(jobs=None, verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    objs = []
+    failed = []
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for src, rc, out in ex.map(_compile, SOURCES):
+            objs.append(os.path.join(BUILD, os.path.basename(src) + ".o"))
+            if rc != 0:
+                failed.append((src, out))
+            elif verbose and out.strip():
+                print(f"[{src}] {out}")
+    if failed:
+        for src, out in failed:
+            print(f"--- compile failed: {src}\n{out}", file=sys.stderr)
+        raise RuntimeError("libdim_gpu build failed: " + ", ".join(s for s, _ in failed))
+    tmp = LIB + ".tmp"
+    rc, out = _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
+    if rc != 0:
+        raise RuntimeError("libdim_gpu link failed:\n" + out)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_cli():
+    src = os.path.join(PKG, "cli", "dim_cli.cpp")
+    if not os.path.exists(src):
+        return None
+    inc = os.path.join(HDF5_ROOT, "include")
+    libdir = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"HDF5 headers not found under {HDF5_ROOT}; skipping the dim CLI", file=sys.stderr)
+        return None
+    tmp = CLI + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", inc, "-I", os.path.join(ROOT, "include"), src, "-o", tmp,
+           "-L", libdir, "-lhdf5", f"-Wl,-rpath,{libdir}", "-L", PKG, "-ldim_gpu", "-Wl,-rpath,$ORIGIN"]
+    rc, out = _run(cmd)
+    if rc != 0:
+        raise RuntimeError("dim CLI build failed:\n" + out)
+    os.replace(tmp, CLI)
+    return CLI
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib", action="store_true", help="library only")
+    ap.add_argument("--clean", action="store_true")
+    ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("-v", action="store_true")
+    a = ap.parse_args(argv)
+    if a.clean and os.path.isdir(BUILD):
+        shutil.rmtree(BUILD)
+    print(build_lib(a.j, a.v))
+    if not a.lib:
+        print(build_cli())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,439 @@
+// C ABI of libdim_gpu.so: matrix construction, products and the autotuner.
+// The CG entry points live in cg.hip.
+#include "handle.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+
+namespace dimgpu {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+dim_status fail(dim_status st, const std::string& msg) {
+    set_error(msg);
+    return st;
+}
+
+dim_status hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? DIM_OUT_OF_MEMORY : DIM_HIP_ERROR;
+}
+
+void free_handle(dim_csr5* h) {
+    if (!h)
+        return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(h->device);
+    for (void* p : h->allocations)
+        (void)hipFree(p);
+    (void)hipSetDevice(prev);
+    delete h;
+}
+
+dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out) {
+    std::unique_ptr<dim_csr5, void (*)(dim_csr5*)> h(new dim_csr5, free_handle);
+    DIM_HIP_TRY(hipGetDevice(&h->device));
+    DevFormat& d = h->f;
+    d.rows = f.rows;
+    d.cols = f.cols;
+    d.sigma = f.sigma;
+    d.tiles = f.tiles;
+    d.nnz = f.nnz;
+    d.padded = f.padded;
+    d.any_dirty = f.any_dirty;
+    d.n_empty = static_cast<uint32_t>(f.empty_rows.size());
+    DIM_HIP_TRY(upload(h.get(), d.tile_ptr, f.tile_ptr));
+    if (f.sigma <= 16) {
+        std::vector<uint16_t> narrow(f.bit_flag.begin(), f.bit_flag.end());
+        uint16_t* p = nullptr;
+        DIM_HIP_TRY(upload(h.get(), p, narrow));
+        d.bit_flag = p;
+    } else {
+        uint32_t* p = nullptr;
+        DIM_HIP_TRY(upload(h.get(), p, f.bit_flag));
+        d.bit_flag = p;
+    }
+    DIM_HIP_TRY(upload(h.get(), d.tile_flags, f.tile_flags));
+    DIM_HIP_TRY(upload(h.get(), d.tile_desc_offset_ptr, f.tile_desc_offset_ptr));
+    DIM_HIP_TRY(upload(h.get(), d.tile_desc_offset, f.tile_desc_offset));
+    DIM_HIP_TRY(upload(h.get(), d.col_idx, f.col_idx));
+    DIM_HIP_TRY(upload(h.get(), d.vals64, f.vals));
+    if (keep_f32) {
+        std::vector<float> v32(f.vals.begin(), f.vals.end());
+        DIM_HIP_TRY(upload(h.get(), d.vals32, v32));
+    }
+    DIM_HIP_TRY(upload(h.get(), d.empty_rows, f.empty_rows));
+    for (uint32_t t = 0; t < f.tiles; ++t)
+        h->n_dirty += (f.tile_ptr[t] & kDirty) ? 1 : 0;
+
+    Scratch& s = h->s;
+    const size_t T = std::max<size_t>(f.tiles, 1);
+    double* hp = nullptr;
+    double* tp = nullptr;
+    DIM_HIP_TRY(dev_alloc(h.get(), hp, T * kMaxK));
+    DIM_HIP_TRY(dev_alloc(h.get(), tp, T * kMaxK));
+    s.head = hp;
+    s.tail = tp;
+    DIM_HIP_TRY(dev_alloc(h.get(), s.dot_tile, T * kMaxK));
+    s.fix_blocks = static_cast<uint32_t>((T + 255) / 256);
+    DIM_HIP_TRY(dev_alloc(h.get(), s.dot_blk, static_cast<size_t>(s.fix_blocks) * kMaxK));
+    h->sys_row = {0, f.rows};
+    h->sys_tile = {0, f.tiles};
+    h->sys_nnz_csr = {0, static_cast<uint32_t>(f.nnz)};
+    *out = h.release();
+    return DIM_SUCCESS;
+}
+
+namespace {
+
+dim_status check_handle(const dim_csr5* A) {
+    if (!A)
+        return fail(DIM_INVALID_ARGUMENT, "matrix handle is null");
+    return DIM_SUCCESS;
+}
+
+// Times candidate tile heights on this matrix and keeps the fastest:
+// sigma trades per-tile fixed work (descriptor decode, wavefront scan, carry
+// slots) against registers per lane and load-balance granularity.
+dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
+                          const uint32_t* col_idx, const double* vals, bool keep_f32, dim_csr5** out) {
+    static const uint32_t candidates[] = {8, 16, 24, 32};
+    dim_csr5* best = nullptr;
+    float best_ms = 0;
+    double* x = nullptr;
+    double* y = nullptr;
+    DIM_HIP_TRY(hipMalloc(&x, std::max<size_t>(cols, 1) * sizeof(double)));
+    if (hipMalloc(&y, std::max<size_t>(rows, 1) * sizeof(double)) != hipSuccess) {
+        (void)hipFree(x);
+        return fail(DIM_OUT_OF_MEMORY, "autotune: cannot allocate y");
+    }
+    {
+        std::vector<double> hx(std::max<size_t>(cols, 1));
+        std::mt19937_64 rng(12345);
+        std::uniform_real_distribution<double> U(-1.0, 1.0);
+        for (auto& v : hx)
+            v = U(rng);
+        (void)hipMemcpy(x, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice);
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    dim_status st = DIM_SUCCESS;
+    for (uint32_t sigma : candidates) {
+        HostFormat f;
+        std::string err = build_format(rows, cols, nnz, row_ptr, col_idx, vals, sigma, f);
+        if (!err.empty()) {
+            st = fail(DIM_INVALID_ARGUMENT, err);
+            break;
+        }
+        dim_csr5* h = nullptr;
+        st = upload_format(f, keep_f32, &h);
+        if (st != DIM_SUCCESS)
+            break;
+        for (int i = 0; i < 3; ++i)
+            (void)launch_spmv(h->f, h->s, false, 1, x, y, 0, h->f.tiles, 0, 0, h->f.n_empty, false, nullptr, 0);
+        const int reps = 10;
+        (void)hipEventRecord(e0, 0);
+        for (int i = 0; i < reps; ++i)
+            (void)launch_spmv(h->f, h->s, false, 1, x, y, 0, h->f.tiles, 0, 0, h->f.n_empty, false, nullptr, 0);
+        (void)hipEventRecord(e1, 0);
+        hipError_t e = hipEventSynchronize(e1);
+        if (e != hipSuccess) {
+            free_handle(h);
+            st = hip_fail(e, "autotune timing");
+            break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (!best || ms < best_ms) {
+            free_handle(best);
+            best = h;
+            best_ms = ms;
+        } else {
+            free_handle(h);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(x);
+    (void)hipFree(y);
+    if (st != DIM_SUCCESS) {
+        free_handle(best);
+        return st;
+    }
+    *out = best;
+    return DIM_SUCCESS;
+}
+
+} // namespace
+} // namespace dimgpu
+
+using namespace dimgpu;
+
+extern "C" {
+
+const char* dim_last_error(void) { return g_last_error.c_str(); }
+
+const char* dim_status_string(dim_status s) {
+    switch (s) {
+        case DIM_SUCCESS: return "success";
+        case DIM_INVALID_ARGUMENT: return "invalid argument";
+        case DIM_HIP_ERROR: return "HIP error";
+        case DIM_OUT_OF_MEMORY: return "out of memory";
+        case DIM_NOT_CONVERGED: return "not converged";
+        case DIM_BREAKDOWN: return "breakdown (matrix not SPD)";
+        case DIM_IO_ERROR: return "I/O error";
+        case DIM_UNSUPPORTED: return "unsupported";
+    }
+    return "unknown status";
+}
+
+int dim_version(void) { return DIM_GPU_VERSION; }
+
+dim_status dim_csr5_from_csr(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
+                             const uint32_t* col_idx, const double* vals, const dim_csr5_opts* opts,
+                             dim_csr5** out) {
+    if (!out)
+        return fail(DIM_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    if (nnz && !vals)
+        return fail(DIM_INVALID_ARGUMENT, "vals is null");
+    std::string err = validate_csr(rows, cols, nnz, row_ptr, col_idx);
+    if (!err.empty())
+        return fail(DIM_INVALID_ARGUMENT, err);
+    const uint32_t sigma = opts ? opts->sigma : 16;
+    const bool keep_f32 = opts && (opts->flags & DIM_CSR5_KEEP_F32);
+    if (sigma == 0)
+        return autotune_sigma(rows, cols, nnz, row_ptr, col_idx, vals, keep_f32, out);
+    HostFormat f;
+    err = build_format(rows, cols, nnz, row_ptr, col_idx, vals, sigma, f);
+    if (!err.empty())
+        return fail(DIM_INVALID_ARGUMENT, err);
+    return upload_format(f, keep_f32, out);
+}
+
+dim_status dim_csr5_destroy(dim_csr5* A) {
+    free_handle(A);
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_get_info(const dim_csr5* A, dim_csr5_info* info) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (!info)
+        return fail(DIM_INVALID_ARGUMENT, "info is null");
+    info->rows = A->f.rows;
+    info->cols = A->f.cols;
+    info->nnz = A->f.nnz;
+    info->sigma = A->f.sigma;
+    info->omega = kOmega;
+    info->tile_count = A->f.tiles;
+    info->tail_partition_start = A->f.rows;
+    info->n_empty_rows = A->f.n_empty;
+    info->n_dirty_tiles = A->n_dirty;
+    info->has_f32 = A->f.vals32 != nullptr;
+    info->device_bytes = A->device_bytes;
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_copy_format(const dim_csr5* A, uint32_t* tile_ptr, uint32_t* bit_flag, uint8_t* tile_flags,
+                                uint32_t* tile_desc_offset_ptr, uint32_t* tile_desc_offset, uint64_t* n_tdo,
+                                uint32_t* empty_rows) {
+    if (dim_status st = check_handle(A))
+        return st;
+    const DevFormat& d = A->f;
+    const size_t T = d.tiles;
+    if (tile_ptr)
+        DIM_HIP_TRY(hipMemcpy(tile_ptr, d.tile_ptr, (T + 1) * 4, hipMemcpyDeviceToHost));
+    if (bit_flag && T) {
+        if (d.sigma <= 16) {
+            std::vector<uint16_t> tmp(T * kOmega);
+            DIM_HIP_TRY(hipMemcpy(tmp.data(), d.bit_flag, tmp.size() * 2, hipMemcpyDeviceToHost));
+            std::copy(tmp.begin(), tmp.end(), bit_flag);
+        } else {
+            DIM_HIP_TRY(hipMemcpy(bit_flag, d.bit_flag, T * kOmega * 4, hipMemcpyDeviceToHost));
+        }
+    }
+    if (tile_flags)
+        DIM_HIP_TRY(hipMemcpy(tile_flags, d.tile_flags, T + 1, hipMemcpyDeviceToHost));
+    uint32_t ntdo = 0;
+    DIM_HIP_TRY(hipMemcpy(&ntdo, d.tile_desc_offset_ptr + T, 4, hipMemcpyDeviceToHost));
+    if (tile_desc_offset_ptr)
+        DIM_HIP_TRY(hipMemcpy(tile_desc_offset_ptr, d.tile_desc_offset_ptr, (T + 1) * 4, hipMemcpyDeviceToHost));
+    if (tile_desc_offset && ntdo)
+        DIM_HIP_TRY(hipMemcpy(tile_desc_offset, d.tile_desc_offset, size_t{ntdo} * 4, hipMemcpyDeviceToHost));
+    if (n_tdo)
+        *n_tdo = ntdo;
+    if (empty_rows && d.n_empty)
+        DIM_HIP_TRY(hipMemcpy(empty_rows, d.empty_rows, size_t{d.n_empty} * 4, hipMemcpyDeviceToHost));
+    return DIM_SUCCESS;
+}
+
+static dim_status check_dtype(const dim_csr5* A, dim_dtype dtype) {
+    if (dtype != DIM_F64 && dtype != DIM_F32)
+        return fail(DIM_INVALID_ARGUMENT, "dtype must be DIM_F64 or DIM_F32");
+    if (dtype == DIM_F32 && !A->f.vals32)
+        return fail(DIM_INVALID_ARGUMENT, "matrix was built without DIM_CSR5_KEEP_F32");
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_spmv(const dim_csr5* A, dim_dtype dtype, const void* x, uint64_t x_len, void* y,
+                         uint64_t y_len, void* stream) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (dim_status st = check_dtype(A, dtype))
+        return st;
+    if (x_len != A->f.cols)
+        return fail(DIM_INVALID_ARGUMENT, "can't multiply, vector size doesn't match column count of the matrix");
+    if (y_len != A->f.rows)
+        return fail(DIM_INVALID_ARGUMENT, "output vector size doesn't match row count of the matrix");
+    if ((A->f.cols && !x) || (A->f.rows && !y))
+        return fail(DIM_INVALID_ARGUMENT, "null vector");
+    auto* self = const_cast<dim_csr5*>(A);
+    DIM_HIP_TRY(launch_spmv(self->f, self->s, dtype == DIM_F32, 1, x, y, 0, A->f.tiles, 0, 0, A->f.n_empty, false,
+                            nullptr, static_cast<hipStream_t>(stream)));
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_spmv_timed(const dim_csr5* A, dim_dtype dtype, const void* x, uint64_t x_len, void* y,
+                               uint64_t y_len, void* stream, float* ms) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (dim_status st = check_dtype(A, dtype))
+        return st;
+    if (x_len != A->f.cols || y_len != A->f.rows || !ms)
+        return fail(DIM_INVALID_ARGUMENT, "vector size mismatch or null ms");
+    hipEvent_t ev[3] = {};
+    for (auto& e : ev)
+        DIM_HIP_TRY(hipEventCreate(&e));
+    auto* self = const_cast<dim_csr5*>(A);
+    hipError_t e = launch_spmv_events(self->f, self->s, dtype == DIM_F32, x, y, ev[0], ev[1], ev[2],
+                                      static_cast<hipStream_t>(stream));
+    if (e == hipSuccess)
+        e = hipEventSynchronize(ev[2]);
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&ms[0], ev[0], ev[1]);
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&ms[1], ev[1], ev[2]);
+    for (auto& ee : ev)
+        (void)hipEventDestroy(ee);
+    if (e != hipSuccess)
+        return hip_fail(e, "dim_csr5_spmv_timed");
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_block_rows(const dim_csr5* A, uint32_t tile_begin, uint32_t tile_end, uint32_t* first_row,
+                               uint32_t* n_rows) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (tile_begin >= tile_end || tile_end > A->f.tiles)
+        return fail(DIM_INVALID_ARGUMENT, "tile range must satisfy tile_begin < tile_end <= tile_count");
+    uint32_t tp[2];
+    DIM_HIP_TRY(hipMemcpy(&tp[0], A->f.tile_ptr + tile_begin, 4, hipMemcpyDeviceToHost));
+    DIM_HIP_TRY(hipMemcpy(&tp[1], A->f.tile_ptr + tile_end, 4, hipMemcpyDeviceToHost));
+    const uint32_t first = tp[0] & ~kDirty;
+    // the block's last row: the row holding its last element
+    uint32_t last;
+    if (tile_end == A->f.tiles) {
+        last = A->f.rows ? A->f.rows - 1 : 0;
+    } else {
+        uint8_t fl = 0;
+        DIM_HIP_TRY(hipMemcpy(&fl, A->f.tile_flags + tile_end, 1, hipMemcpyDeviceToHost));
+        const uint32_t next = tp[1] & ~kDirty;
+        last = (fl & kStartsRow) ? next - 1 : next;
+    }
+    if (first_row)
+        *first_row = first;
+    if (n_rows)
+        *n_rows = last >= first ? last - first + 1 : 0;
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_spmv_block(const dim_csr5* A, dim_dtype dtype, uint32_t tile_begin, uint32_t tile_end,
+                               const void* x, uint64_t x_len, void* y_block, uint64_t y_len, void* stream) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (dim_status st = check_dtype(A, dtype))
+        return st;
+    if (x_len != A->f.cols)
+        return fail(DIM_INVALID_ARGUMENT, "can't multiply, vector size doesn't match column count of the matrix");
+    uint32_t first = 0, n = 0;
+    if (dim_status st = dim_csr5_block_rows(A, tile_begin, tile_end, &first, &n))
+        return st;
+    if (y_len != n)
+        return fail(DIM_INVALID_ARGUMENT, "y_block must have last_row - first_row + 1 entries");
+    // empty rows inside [first, first + n)
+    std::vector<uint32_t> er(A->f.n_empty);
+    if (A->f.n_empty)
+        DIM_HIP_TRY(hipMemcpy(er.data(), A->f.empty_rows, er.size() * 4, hipMemcpyDeviceToHost));
+    const uint32_t eb = static_cast<uint32_t>(std::lower_bound(er.begin(), er.end(), first) - er.begin());
+    const uint32_t ee = static_cast<uint32_t>(std::lower_bound(er.begin(), er.end(), first + n) - er.begin());
+    auto* self = const_cast<dim_csr5*>(A);
+    DIM_HIP_TRY(launch_spmv(self->f, self->s, dtype == DIM_F32, 1, x, y_block, tile_begin, tile_end, first, eb, ee,
+                            false, nullptr, static_cast<hipStream_t>(stream)));
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_spmm(const dim_csr5* A, dim_dtype dtype, uint32_t k, const void* X, void* Y, void* stream) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (dim_status st = check_dtype(A, dtype))
+        return st;
+    if (k < 1 || k > static_cast<uint32_t>(kMaxK))
+        return fail(DIM_INVALID_ARGUMENT, "k must be in 1..16");
+    if ((A->f.cols && !X) || (A->f.rows && !Y))
+        return fail(DIM_INVALID_ARGUMENT, "null matrix argument");
+    auto* self = const_cast<dim_csr5*>(A);
+    DIM_HIP_TRY(launch_spmv(self->f, self->s, dtype == DIM_F32, static_cast<int>(k), X, Y, 0, A->f.tiles, 0, 0,
+                            A->f.n_empty, false, nullptr, static_cast<hipStream_t>(stream)));
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_spmv_transpose(const dim_csr5* A, dim_dtype dtype, const void* x, void* y, void* stream) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (dim_status st = check_dtype(A, dtype))
+        return st;
+    if ((A->f.rows && !x) || (A->f.cols && !y))
+        return fail(DIM_INVALID_ARGUMENT, "null vector");
+    DIM_HIP_TRY(launch_spmv_transpose(A->f, dtype == DIM_F32, x, y, static_cast<hipStream_t>(stream)));
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_sampled_outer(const dim_csr5* A, double scale, const double* u, const double* v, double* out,
+                                  void* stream) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if ((A->f.rows && !u) || (A->f.cols && !v) || (A->f.nnz && !out))
+        return fail(DIM_INVALID_ARGUMENT, "null vector");
+    for (uint32_t s = 0; s < A->n_systems; ++s) {
+        DevFormat view = A->f;
+        const uint32_t t0 = A->sys_tile[s], t1 = A->sys_tile[s + 1];
+        const size_t tile = size_t{kOmega} * A->f.sigma;
+        view.tiles = t1 - t0;
+        view.tile_ptr = A->f.tile_ptr + t0;
+        view.bit_flag = static_cast<char*>(A->f.bit_flag) + size_t{t0} * kOmega * (A->f.sigma <= 16 ? 2 : 4);
+        view.tile_flags = A->f.tile_flags + t0;
+        view.tile_desc_offset_ptr = A->f.tile_desc_offset_ptr + t0;
+        view.col_idx = A->f.col_idx + t0 * tile;
+        view.vals64 = A->f.vals64 + t0 * tile;
+        view.nnz = A->sys_nnz_csr[s + 1] - A->sys_nnz_csr[s];
+        DIM_HIP_TRY(launch_sampled_outer(view, scale, u, v, out + A->sys_nnz_csr[s], static_cast<hipStream_t>(stream)));
+    }
+    return DIM_SUCCESS;
+}
+
+uint32_t dim_csr5_n_systems(const dim_csr5* A) { return A ? A->n_systems : 0; }
+
+} // extern "C"

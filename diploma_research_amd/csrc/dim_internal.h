@@ -1,0 +1,60 @@
+// Internal declarations shared by the host format builder, the HIP kernels
+// and the C ABI of libdim_gpu.so.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dimgpu {
+
+// One CSR5 tile column is one lane of a 64-wide wavefront.
+constexpr uint32_t kOmega = 64;
+// tile_ptr MSB: the tile's rows include an empty row, so its segments map to
+// rows through tile_desc_offset (same convention as the reference,
+// include/dim/mat/storage_formats/csr5.h:156-169, 342-397).
+constexpr uint32_t kDirty = 0x80000000u;
+// tile_flags bits.
+constexpr uint8_t kStartsRow = 1; // the tile's first element is the first element of a row
+constexpr uint8_t kMultiSeg = 2;  // the tile holds more than one row segment
+
+inline bool sigma_supported(uint32_t s) { return s == 8 || s == 16 || s == 24 || s == 32; }
+
+// Host image of the GPU CSR5 format (omega = 64, sigma in {8,16,24,32}).
+//
+//  * tiles = ceil(nnz / (64*sigma)); the last tile is padded (val 0, col 0)
+//    instead of keeping the reference's separate CSR tail partition.
+//  * vals/col_idx: inside tile t the CSR element t*64*sigma + c*sigma + r sits
+//    at t*64*sigma + r*64 + c, so step r of a wavefront is one contiguous
+//    512-byte (f64) load.
+//  * bit_flag[t*64 + c]: bit r set iff that element starts a row.
+//  * tile_ptr[t]: row holding the tile's first element (MSB = dirty);
+//    tile_ptr[tiles] = rows.
+//  * tile_desc_offset: for dirty tiles, the row of each segment relative to
+//    tile_ptr (segment 0 begins at the tile's first element).
+//  * empty_rows: rows without non-zeros; the kernels write 0 there.
+struct HostFormat {
+    uint32_t rows = 0, cols = 0, sigma = 16, tiles = 0;
+    uint64_t nnz = 0, padded = 0;
+    std::vector<uint32_t> tile_ptr;
+    std::vector<uint32_t> bit_flag; // widened; narrowed to u16 on upload for sigma <= 16
+    std::vector<uint8_t> tile_flags;
+    std::vector<uint32_t> tile_desc_offset_ptr;
+    std::vector<uint32_t> tile_desc_offset;
+    std::vector<uint32_t> col_idx;
+    std::vector<double> vals;
+    std::vector<uint32_t> empty_rows;
+    bool any_dirty = false;
+};
+
+// Builds the format from CSR. Returns an empty string or an error message.
+std::string build_format(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
+                         const uint32_t* col_idx, const double* vals, uint32_t sigma, HostFormat& out);
+
+// Validates CSR structure (monotone row_ptr, row_ptr[0]==0, row_ptr[rows]==nnz,
+// column indices < cols). Returns an empty string or an error message.
+std::string validate_csr(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
+                         const uint32_t* col_idx);
+
+} // namespace dimgpu

@@ -1,0 +1,73 @@
+// The dim_csr5 handle behind the C ABI, and error plumbing shared by the
+// C ABI translation units.
+#pragma once
+
+#include "dim_internal.h"
+#include "kernels.h"
+#include "../../include/dim_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+struct dim_csr5 {
+    dimgpu::DevFormat f;
+    dimgpu::Scratch s;
+    int device = 0;
+    uint32_t n_dirty = 0;
+    uint64_t device_bytes = 0;
+    std::vector<void*> allocations;
+    // block-diagonal batches: system i owns rows [sys_row[i], sys_row[i+1]) and
+    // tiles [sys_tile[i], sys_tile[i+1])
+    uint32_t n_systems = 1;
+    std::vector<uint32_t> sys_row, sys_tile, sys_nnz_csr;
+    uint32_t* d_sys_row = nullptr;  // n_systems + 1
+    uint32_t* d_sys_tile = nullptr; // n_systems + 1
+    // CSR-order position of every stored element (for batched matrices the
+    // element order of the concatenated per-system CSR arrays)
+};
+
+namespace dimgpu {
+
+void set_error(const std::string& msg);
+dim_status fail(dim_status st, const std::string& msg);
+dim_status hip_fail(hipError_t e, const char* what);
+
+#define DIM_HIP_TRY(expr)                                                                                          \
+    do {                                                                                                           \
+        hipError_t e_ = (expr);                                                                                    \
+        if (e_ != hipSuccess)                                                                                      \
+            return ::dimgpu::hip_fail(e_, #expr);                                                                  \
+    } while (0)
+
+// device allocation owned by the handle
+template<typename T>
+hipError_t dev_alloc(dim_csr5* h, T*& p, size_t count) {
+    p = nullptr;
+    const size_t bytes = count ? count * sizeof(T) : sizeof(T);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return e;
+    h->allocations.push_back(q);
+    h->device_bytes += bytes;
+    p = static_cast<T*>(q);
+    return hipSuccess;
+}
+
+template<typename T>
+hipError_t upload(dim_csr5* h, T*& p, const std::vector<T>& v) {
+    hipError_t e = dev_alloc(h, p, v.size());
+    if (e != hipSuccess)
+        return e;
+    if (!v.empty())
+        e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+// Uploads a host format into a fresh handle (allocates scratch too).
+dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out);
+void free_handle(dim_csr5* h);
+
+} // namespace dimgpu

@@ -1,0 +1,64 @@
+// Device-side view of the GPU CSR5 format and the kernel launch entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dimgpu {
+
+// Device pointers of one matrix (see HostFormat for the layout).
+struct DevFormat {
+    uint32_t rows = 0, cols = 0, sigma = 16, tiles = 0;
+    uint64_t nnz = 0, padded = 0;
+    uint32_t* tile_ptr = nullptr;
+    void* bit_flag = nullptr; // u16 words for sigma <= 16, u32 otherwise
+    uint8_t* tile_flags = nullptr;
+    uint32_t* tile_desc_offset_ptr = nullptr;
+    uint32_t* tile_desc_offset = nullptr;
+    uint32_t* col_idx = nullptr;
+    double* vals64 = nullptr;
+    float* vals32 = nullptr;
+    uint32_t* empty_rows = nullptr;
+    uint32_t n_empty = 0;
+    bool any_dirty = false;
+};
+
+// Per-matrix scratch: tile carries and reduction partials (the GPU
+// counterpart of the reference's calibrator, include/dim/mat/storage_formats/
+// csr5/calibrator.h:14-31). Sized for up to kMaxK right-hand sides.
+constexpr int kMaxK = 16;
+struct Scratch {
+    void* head = nullptr;     // tiles * kMaxK * 8 bytes
+    void* tail = nullptr;     // tiles * kMaxK * 8 bytes
+    double* dot_tile = nullptr; // tiles * kMaxK
+    double* dot_blk = nullptr;  // ceil(tiles/256) * kMaxK
+    uint32_t fix_blocks = 0;
+};
+
+// Which reduction the tile kernels fold into their epilogue.
+enum class Epilogue : int { none = 0, dot_xy = 1 };
+
+// y = A x over tiles [tile_begin, tile_end). x and y are n x k, row-major.
+// row_base: y row index offset (0 for the whole matrix; first row of a block).
+// When `dot` is set (square matrices, whole matrix only) the per-column
+// products x[row]*y[row] are reduced into scratch.dot_blk.
+// `done` (may be null): a device flag; when *done != 0 the kernels return at
+// once (used by batched solver launches).
+hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y,
+                       uint32_t tile_begin, uint32_t tile_end, uint32_t row_base, uint32_t empty_begin,
+                       uint32_t empty_end, bool dot, const int* done, hipStream_t stream);
+
+// Same as launch_spmv over the whole matrix with events recorded before the
+// tile kernel, between the two kernels and after the fix-up kernel.
+hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const void* x, void* y, hipEvent_t e0,
+                              hipEvent_t e1, hipEvent_t e2, hipStream_t stream);
+
+// y = A^T x (x has rows entries, y has cols entries); y is zeroed first.
+hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, void* y, hipStream_t stream);
+
+// out[e] = scale * u[row(e)] * v[col(e)] for every stored element e, in CSR order.
+hipError_t launch_sampled_outer(const DevFormat& A, double scale, const double* u, const double* v, double* out,
+                                hipStream_t stream);
+
+} // namespace dimgpu

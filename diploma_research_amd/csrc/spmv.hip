@@ -1,0 +1,276 @@
+// CSR5 SpMV / SpMM kernels for gfx950 (CDNA4, wave64).
+//
+// One wavefront owns one tile of 64 columns x sigma steps. The reference does
+// the same work with four AVX2 lanes per tile and OpenMP threads over tile
+// chunks (include/dim/mat/storage_formats/csr5.h:619-748); here the 64 tile
+// columns are the 64 lanes, so:
+//   * step r of the tile is one coalesced load of 64 values and 64 column
+//     indices (the per-tile transpose of csr5.h:463-493 puts them side by side);
+//   * the lane's bit flags are decoded in registers, the per-column y_offset is
+//     a wavefront prefix sum of popcounts and scansum_offset is read off a
+//     ballot of the lanes that start a row (csr5.h:74-97);
+//   * rows that cross lanes are closed with a segmented wavefront scan of the
+//     lanes' leading partial sums, which replaces fast_segmented_sum /
+//     compute_last_sum (csr5.h:200-235) and keeps each sum in left-to-right
+//     order instead of differencing prefix sums;
+//   * rows that cross tiles leave their partial sums in per-tile head/tail
+//     slots, and a second kernel adds the chains up in tile order and writes
+//     those rows once. This replaces the reference's per-thread calibrator
+//     (csr5.h:526-538, 750-754) without atomics, so results are bitwise
+//     reproducible from run to run.
+#include "spmv_impl.h"
+
+namespace dimgpu {
+
+namespace {
+
+using namespace detail;
+
+// ------------------------------------------------------ transpose SpMV ---
+
+template<typename V>
+__global__ void k_fill(V* y, size_t n, V value) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        y[i] = value;
+}
+
+// Row of the element at step r of this lane, from the lane's bit flags.
+__device__ __forceinline__ uint32_t element_row(uint32_t bits, int r, int y_off, uint32_t row0, bool dirty,
+                                                const uint32_t* tdo) {
+    const uint32_t upto = (r >= 31) ? 0xFFFFFFFFu : ((2u << r) - 1u);
+    const int s = y_off - 1 + __popc(bits & upto);
+    return row0 + (dirty ? tdo[s] : static_cast<uint32_t>(s));
+}
+
+// y[col] += a_ij * x[row] for every element: the transposed product scatters
+// into y with hardware floating-point atomics.
+template<typename V, int SIGMA>
+__global__ __launch_bounds__(256) void k_csr5_transpose(TileParams p) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= p.tile_end)
+        return;
+    using W = typename DescWord<SIGMA>::type;
+    const V* x = static_cast<const V*>(p.x);
+    V* y = static_cast<V*>(p.y);
+    const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
+    const uint32_t bits = static_cast<uint32_t>(static_cast<const W*>(p.bit_flag)[static_cast<size_t>(t) * 64 + lane]) |
+                          (lane == 0 ? 1u : 0u);
+    const uint32_t tpw = p.tile_ptr[t];
+    const bool dirty = (tpw & kDirty) != 0;
+    const uint32_t row0 = tpw & ~kDirty;
+    const uint32_t* tdo = dirty ? p.tdo + p.tdo_ptr[t] : nullptr;
+    const int nb = __popc(bits);
+    const int y_off = wave_incl_scan(nb, lane) - nb;
+    const V* vals = static_cast<const V*>(p.vals);
+#pragma unroll
+    for (int r = 0; r < SIGMA; ++r) {
+        const size_t pos = base + static_cast<size_t>(r) * 64 + lane;
+        const size_t csr = base + static_cast<size_t>(lane) * SIGMA + r;
+        if (csr >= p.nnz)
+            continue;
+        const uint32_t row = element_row(bits, r, y_off, row0, dirty, tdo);
+        const V a = vals[pos];
+        unsafeAtomicAdd(y + p.col_idx[pos], a * x[row]);
+    }
+}
+
+// out[csr(e)] = scale * u[row(e)] * v[col(e)]
+template<int SIGMA>
+__global__ __launch_bounds__(256) void k_sampled_outer(TileParams p, double scale, const double* u, const double* v,
+                                                       double* out) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= p.tile_end)
+        return;
+    using W = typename DescWord<SIGMA>::type;
+    const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
+    const uint32_t bits = static_cast<uint32_t>(static_cast<const W*>(p.bit_flag)[static_cast<size_t>(t) * 64 + lane]) |
+                          (lane == 0 ? 1u : 0u);
+    const uint32_t tpw = p.tile_ptr[t];
+    const bool dirty = (tpw & kDirty) != 0;
+    const uint32_t row0 = tpw & ~kDirty;
+    const uint32_t* tdo = dirty ? p.tdo + p.tdo_ptr[t] : nullptr;
+    const int nb = __popc(bits);
+    const int y_off = wave_incl_scan(nb, lane) - nb;
+#pragma unroll
+    for (int r = 0; r < SIGMA; ++r) {
+        const size_t pos = base + static_cast<size_t>(r) * 64 + lane;
+        const size_t csr = base + static_cast<size_t>(lane) * SIGMA + r;
+        if (csr >= p.nnz)
+            continue;
+        const uint32_t row = element_row(bits, r, y_off, row0, dirty, tdo);
+        out[csr] = scale * u[row] * v[p.col_idx[pos]];
+    }
+}
+
+TileParams make_tile_params(const DevFormat& A, bool f32) {
+    TileParams tp{};
+    tp.tile_ptr = A.tile_ptr;
+    tp.bit_flag = A.bit_flag;
+    tp.tile_flags = A.tile_flags;
+    tp.tdo_ptr = A.tile_desc_offset_ptr;
+    tp.tdo = A.tile_desc_offset;
+    tp.col_idx = A.col_idx;
+    tp.vals = f32 ? static_cast<const void*>(A.vals32) : static_cast<const void*>(A.vals64);
+    tp.nnz = A.nnz;
+    tp.tile_begin = 0;
+    tp.tile_end = A.tiles;
+    tp.last_tile = A.tiles ? A.tiles - 1 : 0;
+    tp.k = 1;
+    return tp;
+}
+
+} // namespace
+
+using namespace detail;
+
+hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
+                       uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, bool dot,
+                       const int* done, hipStream_t stream) {
+    TileParams tp = make_tile_params(A, f32);
+    tp.x = x;
+    tp.y = y;
+    tp.head = S.head;
+    tp.tail = S.tail;
+    tp.dot_tile = S.dot_tile;
+    tp.done = done;
+    tp.tile_begin = tile_begin;
+    tp.tile_end = tile_end;
+    tp.row_base = row_base;
+    tp.k = k;
+
+    FixParams fp{};
+    fp.tile_ptr = A.tile_ptr;
+    fp.tile_flags = A.tile_flags;
+    fp.empty_rows = A.empty_rows;
+    fp.x = x;
+    fp.y = y;
+    fp.head = S.head;
+    fp.tail = S.tail;
+    fp.dot_tile = S.dot_tile;
+    fp.dot_blk = S.dot_blk;
+    fp.done = done;
+    fp.tile_begin = tile_begin;
+    fp.tile_end = tile_end;
+    fp.row_base = row_base;
+    fp.empty_begin = empty_begin;
+    fp.empty_end = empty_end;
+    fp.k = k;
+
+    const uint32_t ntiles = tile_end - tile_begin;
+    const uint32_t tgrid = (ntiles + 3) / 4;
+    const uint32_t fgrid = ntiles ? (ntiles + 255) / 256 : 1;
+    if (k == 1) {
+        if (f32)
+            launch_pair<float, 1>(A, tp, fp, tgrid, fgrid, dot, stream);
+        else
+            launch_pair<double, 1>(A, tp, fp, tgrid, fgrid, dot, stream);
+    } else if (k <= 2) {
+        launch_pair_k2(f32, A, tp, fp, tgrid, fgrid, dot, stream);
+    } else if (k <= 4) {
+        launch_pair_k4(f32, A, tp, fp, tgrid, fgrid, dot, stream);
+    } else if (k <= 8) {
+        launch_pair_k8(f32, A, tp, fp, tgrid, fgrid, dot, stream);
+    } else {
+        launch_pair_k16(f32, A, tp, fp, tgrid, fgrid, dot, stream);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const void* x, void* y, hipEvent_t e0,
+                              hipEvent_t e1, hipEvent_t e2, hipStream_t stream) {
+    TileParams tp = make_tile_params(A, f32);
+    tp.x = x;
+    tp.y = y;
+    tp.head = S.head;
+    tp.tail = S.tail;
+    tp.dot_tile = S.dot_tile;
+    FixParams fp{};
+    fp.tile_ptr = A.tile_ptr;
+    fp.tile_flags = A.tile_flags;
+    fp.empty_rows = A.empty_rows;
+    fp.x = x;
+    fp.y = y;
+    fp.head = S.head;
+    fp.tail = S.tail;
+    fp.dot_tile = S.dot_tile;
+    fp.dot_blk = S.dot_blk;
+    fp.tile_end = A.tiles;
+    fp.empty_end = A.n_empty;
+    fp.k = 1;
+    const uint32_t tgrid = (A.tiles + 3) / 4;
+    const uint32_t fgrid = A.tiles ? (A.tiles + 255) / 256 : 1;
+    hipError_t e = hipEventRecord(e0, stream);
+    if (e != hipSuccess)
+        return e;
+    if (tgrid) {
+        if (f32)
+            launch_by_sigma<float, 1, false>(A.sigma, tp, tgrid, stream);
+        else
+            launch_by_sigma<double, 1, false>(A.sigma, tp, tgrid, stream);
+    }
+    e = hipEventRecord(e1, stream);
+    if (e != hipSuccess)
+        return e;
+    if (f32)
+        hipLaunchKernelGGL((k_csr5_fixup<float, 1, false>), dim3(fgrid), dim3(256), 0, stream, fp);
+    else
+        hipLaunchKernelGGL((k_csr5_fixup<double, 1, false>), dim3(fgrid), dim3(256), 0, stream, fp);
+    e = hipEventRecord(e2, stream);
+    if (e != hipSuccess)
+        return e;
+    return hipGetLastError();
+}
+
+hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, void* y, hipStream_t stream) {
+    TileParams tp = make_tile_params(A, f32);
+    tp.x = x;
+    tp.y = y;
+    const uint32_t fill_grid = 1024;
+    if (f32)
+        hipLaunchKernelGGL(k_fill<float>, dim3(fill_grid), dim3(256), 0, stream, static_cast<float*>(y),
+                           static_cast<size_t>(A.cols), 0.0f);
+    else
+        hipLaunchKernelGGL(k_fill<double>, dim3(fill_grid), dim3(256), 0, stream, static_cast<double*>(y),
+                           static_cast<size_t>(A.cols), 0.0);
+    if (A.tiles) {
+        const uint32_t grid = (A.tiles + 3) / 4;
+#define DIM_T(V, S) hipLaunchKernelGGL((k_csr5_transpose<V, S>), dim3(grid), dim3(256), 0, stream, tp)
+        if (f32) {
+            switch (A.sigma) {
+                case 8: DIM_T(float, 8); break;
+                case 16: DIM_T(float, 16); break;
+                case 24: DIM_T(float, 24); break;
+                default: DIM_T(float, 32); break;
+            }
+        } else {
+            switch (A.sigma) {
+                case 8: DIM_T(double, 8); break;
+                case 16: DIM_T(double, 16); break;
+                case 24: DIM_T(double, 24); break;
+                default: DIM_T(double, 32); break;
+            }
+        }
+#undef DIM_T
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sampled_outer(const DevFormat& A, double scale, const double* u, const double* v, double* out,
+                                hipStream_t stream) {
+    if (!A.tiles)
+        return hipSuccess;
+    TileParams tp = make_tile_params(A, false);
+    const uint32_t grid = (A.tiles + 3) / 4;
+    switch (A.sigma) {
+        case 8: hipLaunchKernelGGL((k_sampled_outer<8>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
+        case 16: hipLaunchKernelGGL((k_sampled_outer<16>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
+        case 24: hipLaunchKernelGGL((k_sampled_outer<24>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
+        default: hipLaunchKernelGGL((k_sampled_outer<32>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
+    }
+    return hipGetLastError();
+}
+
+} // namespace dimgpu

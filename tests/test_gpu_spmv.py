@@ -1,0 +1,243 @@
+"""GPU parity of the CSR5 product (HIP kernels via the C ABI) against the
+oracle and the reference's golden fixtures."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from gpu_util import assert_spmv_close, to_dev
+from diploma_research_amd import Csr5, matgen
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+SIGMAS = [8, 16, 24, 32]
+
+
+@pytest.mark.parametrize("name", ["mm", "rand_dirty", "rand_longrows", "rect_exact"])
+@pytest.mark.parametrize("sigma", SIGMAS)
+def test_golden_fixtures(cuda, name, sigma):
+    g = load_golden(name)
+    rows, cols = int(g["rows"]), int(g["cols"])
+    A = Csr5.from_csr((rows, cols, g["row_ptr"], g["col_idx"], g["vals"]), sigma=sigma)
+    y = A.spmv(to_dev(g["x"])).cpu().numpy()
+    assert_spmv_close(y, g["y_ref_t8"], g["row_ptr"], g["col_idx"], g["vals"], g["x"])
+
+
+@pytest.mark.parametrize("sigma", SIGMAS)
+def test_poisson2d_fixture(cuda, sigma):
+    g = load_golden("poisson2d_256")
+    rp, ci, v = matgen.poisson2d(256)
+    x = np.random.default_rng(int(g["x_seed"])).uniform(-1.0, 1.0, 65536)
+    A = Csr5.from_csr((65536, 65536, rp, ci, v), sigma=sigma)
+    y = A.spmv(to_dev(x)).cpu().numpy()
+    assert_spmv_close(y, g["y_ref_t8"], rp, ci, v, x)
+
+
+CASES = [
+    dict(rows=1000, cols=1000, max_len=9, empty_frac=0.15, long_rows=[(3, 300), (500, 700)]),
+    dict(rows=300, cols=5000, max_len=4, empty_frac=0.0, long_rows=[(10, 4000), (11, 64), (200, 1000)]),
+    dict(rows=5000, cols=300, max_len=3, empty_frac=0.6, long_rows=[]),
+    dict(rows=1, cols=20000, max_len=0, empty_frac=0.0, long_rows=[(0, 20000)]),
+    dict(rows=20000, cols=1, max_len=1, empty_frac=0.0, long_rows=[]),
+    dict(rows=4096, cols=4096, max_len=64, empty_frac=0.02, long_rows=[(0, 4096), (4095, 4096)]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("sigma", SIGMAS)
+def test_integer_inputs_bit_exact(cuda, case, sigma):
+    """Integer-valued A and x: every order of summation is exact, so the GPU
+    result must equal the oracle bit for bit (empty rows, long rows, dirty and
+    fast-track tiles, padded last tile, rectangular shapes)."""
+    c = CASES[case]
+    rp, ci, v = matgen.random_csr(c["rows"], c["cols"], seed=100 + case, max_len=c["max_len"],
+                                  empty_frac=c["empty_frac"], long_rows=c["long_rows"], integer=True)
+    x = np.random.default_rng(case).integers(-7, 8, c["cols"]).astype(np.float64)
+    A = Csr5.from_csr((c["rows"], c["cols"], rp, ci, v), sigma=sigma)
+    yg = A.spmv(to_dev(x)).cpu().numpy()
+    ref = O.Csr5(c["rows"], c["cols"], rp, ci, v).spmv(x, nthreads=4)
+    np.testing.assert_array_equal(yg, ref)
+
+
+@pytest.mark.parametrize("sigma", SIGMAS)
+def test_format_matches_oracle_conversion(cuda, sigma):
+    """The GPU format's tile pointers, bit flags and segment->row maps equal
+    the oracle's conversion at omega=64 on every full tile."""
+    rp, ci, v = matgen.random_csr(3000, 3000, seed=sigma, max_len=20, empty_frac=0.1, long_rows=[(5, 2900)])
+    A = Csr5.from_csr((3000, 3000, rp, ci, v), sigma=sigma)
+    f = A.format_arrays()
+    o = O.Csr5(3000, 3000, rp, ci, v, sigma, 64).info
+    T = o["tile_count"]  # full tiles
+    np.testing.assert_array_equal(f["tile_ptr"][:T], o["tile_ptr"][:T])
+    np.testing.assert_array_equal(f["bit_flag"][:T], o["bit_flag"][:T].astype(np.uint32))
+    # segment k of a dirty tile maps to the same relative row in both formats
+    for t in range(T):
+        if o["tile_ptr"][t] & O.DIRTY_BIT:
+            a = o["tile_desc_offset"][o["tile_desc_offset_ptr"][t]:]
+            b = f["tile_desc_offset"][f["tile_desc_offset_ptr"][t]:f["tile_desc_offset_ptr"][t + 1]]
+            np.testing.assert_array_equal(b, a[: len(b)])
+    np.testing.assert_array_equal(f["empty_rows"], np.nonzero(np.diff(rp.astype(np.int64)) == 0)[0])
+
+
+def test_overwrites_y_and_is_deterministic(cuda):
+    import torch
+
+    rp, ci, v = matgen.random_csr(5000, 5000, seed=3, max_len=30, empty_frac=0.2, long_rows=[(9, 4000)])
+    x = np.random.default_rng(0).uniform(-1, 1, 5000)
+    A = Csr5.from_csr((5000, 5000, rp, ci, v), sigma=16)
+    xd = to_dev(x)
+    y1 = torch.full((5000,), float("nan"), dtype=torch.float64, device="cuda")
+    A.spmv(xd, y1)
+    y2 = torch.full((5000,), 123.0, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        A.spmv(xd, y2)
+    assert torch.equal(y1, y2)
+    assert_spmv_close(y1.cpu().numpy(), O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
+
+
+def test_size_mismatch_raises_invalid_argument(cuda):
+    import torch
+
+    rp, ci, v = matgen.random_csr(100, 80, 0)
+    A = Csr5.from_csr((100, 80, rp, ci, v))
+    with pytest.raises(ValueError, match="column count"):
+        A.spmv(torch.zeros(81, dtype=torch.float64, device="cuda"))
+
+
+def test_empty_matrix_and_all_empty_rows(cuda):
+    import torch
+
+    A = Csr5.from_csr((7, 5, np.zeros(8, np.uint32), np.zeros(0, np.uint32), np.zeros(0)))
+    y = A.spmv(torch.ones(5, dtype=torch.float64, device="cuda"), torch.full((7,), 9.0, dtype=torch.float64,
+                                                                             device="cuda"))
+    assert torch.equal(y.cpu(), torch.zeros(7, dtype=torch.float64))
+
+
+@pytest.mark.parametrize("sigma", [8, 16, 32])
+def test_fp32_product_error_bound(cuda, sigma):
+    """FP32 values and vectors: no FP32 reference exists (the reference is
+    FP64-only), so the error is measured against the FP64 oracle and must stay
+    within the FP32 rounding bound (len_i + 2) * u32 * sum |a x| plus the
+    input rounding of A and x."""
+    import torch
+
+    rp, ci, v = matgen.random_csr(4000, 4000, seed=sigma, max_len=40, empty_frac=0.05, long_rows=[(1, 3000)])
+    x = np.random.default_rng(1).uniform(-1, 1, 4000)
+    A = Csr5.from_csr((4000, 4000, rp, ci, v), sigma=sigma, keep_f32=True)
+    y32 = A.spmv(to_dev(x, torch.float32)).double().cpu().numpy()
+    ref = O.csr_spmv(rp, ci, v, x)
+    u32 = np.finfo(np.float32).eps / 2
+    lens = np.diff(rp.astype(np.int64))
+    rows = np.repeat(np.arange(4000), lens)
+    absum = np.bincount(rows, weights=np.abs(v * x[ci]), minlength=4000)
+    assert (np.abs(y32 - ref) <= (lens + 4) * u32 * absum + 1e-30).all()
+    rel = np.linalg.norm(y32 - ref) / np.linalg.norm(ref)
+    assert rel < 1e-5
+
+
+def test_fp32_requires_keep_f32(cuda):
+    import torch
+
+    rp, ci, v = matgen.random_csr(100, 100, 0)
+    A = Csr5.from_csr((100, 100, rp, ci, v))
+    with pytest.raises(ValueError, match="KEEP_F32"):
+        A.spmv(torch.zeros(100, dtype=torch.float32, device="cuda"))
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8, 11, 16])
+@pytest.mark.parametrize("sigma", [8, 16, 32])
+def test_spmm_equals_columnwise_spmv(cuda, k, sigma):
+    """One tile decode for k right-hand sides: integer inputs make it exact."""
+    import torch
+
+    rp, ci, v = matgen.random_csr(3000, 2500, seed=k, max_len=25, empty_frac=0.1, long_rows=[(4, 2400)],
+                                  integer=True)
+    X = np.random.default_rng(k).integers(-4, 5, (2500, k)).astype(np.float64)
+    A = Csr5.from_csr((3000, 2500, rp, ci, v), sigma=sigma, keep_f32=True)
+    Y = A.spmm(to_dev(X)).cpu().numpy()
+    for j in range(k):
+        np.testing.assert_array_equal(Y[:, j], O.csr_spmv(rp, ci, v, X[:, j].copy()))
+    Y32 = A.spmm(to_dev(X, torch.float32)).double().cpu().numpy()
+    np.testing.assert_array_equal(Y32, Y)  # small integers are exact in FP32 too
+
+
+@pytest.mark.parametrize("sigma", SIGMAS)
+def test_transpose_product(cuda, sigma):
+    rp, ci, v = matgen.random_csr(2000, 1700, seed=sigma, max_len=15, empty_frac=0.1, long_rows=[(2, 1600)],
+                                  integer=True)
+    x = np.random.default_rng(2).integers(-5, 6, 2000).astype(np.float64)
+    A = Csr5.from_csr((2000, 1700, rp, ci, v), sigma=sigma)
+    y = A.spmv_transpose(to_dev(x)).cpu().numpy()
+    rows = np.repeat(np.arange(2000), np.diff(rp.astype(np.int64)))
+    exp = np.bincount(ci, weights=v * x[rows], minlength=1700)
+    np.testing.assert_array_equal(y, exp)
+
+
+@pytest.mark.parametrize("sigma", SIGMAS)
+def test_sampled_outer_product(cuda, sigma):
+    rp, ci, v = matgen.random_csr(1500, 1200, seed=sigma, max_len=15, empty_frac=0.1, long_rows=[(2, 1100)])
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal(1500)
+    w = rng.standard_normal(1200)
+    A = Csr5.from_csr((1500, 1200, rp, ci, v), sigma=sigma)
+    out = A.sampled_outer(to_dev(u), to_dev(w), scale=-1.0).cpu().numpy()
+    rows = np.repeat(np.arange(1500), np.diff(rp.astype(np.int64)))
+    np.testing.assert_array_equal(out, -1.0 * u[rows] * w[ci])
+
+
+@pytest.mark.parametrize("sigma", [8, 16, 32])
+def test_row_block_shares(cuda, sigma):
+    """spmv<partial> on tile ranges: each block equals the oracle's block
+    product, and the blocks add up to the full product."""
+    rp, ci, v = matgen.random_csr(6000, 6000, seed=sigma, max_len=30, empty_frac=0.1, long_rows=[(50, 5000)])
+    x = np.random.default_rng(3).uniform(-1, 1, 6000)
+    A = Csr5.from_csr((6000, 6000, rp, ci, v), sigma=sigma)
+    T = A.tile_count
+    cuts = sorted(set([0, T // 3, T // 2, T - 1, T]))
+    acc = np.zeros(6000)
+    tile = 64 * sigma
+    xd = to_dev(x)
+    for b, e in zip(cuts[:-1], cuts[1:]):
+        first, yb = A.spmv_block(xd, b, e)
+        yb = yb.cpu().numpy()
+        ob = O.csr_block_spmv(rp, ci, v, b * tile, min(e * tile, len(v)), x, first, len(yb))
+        np.testing.assert_allclose(yb, ob, rtol=1e-13, atol=1e-13)
+        acc[first: first + len(yb)] += yb
+    np.testing.assert_allclose(acc, O.csr_spmv(rp, ci, v, x), rtol=1e-12, atol=1e-12)
+
+
+def test_autotuned_sigma(cuda):
+    rp, ci, v = matgen.poisson2d(128)
+    A = Csr5.from_csr((16384, 16384, rp, ci, v), sigma=0)
+    assert A.sigma in (8, 16, 24, 32)
+    x = np.random.default_rng(0).uniform(-1, 1, 16384)
+    assert_spmv_close(A.spmv(to_dev(x)).cpu().numpy(), O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
+
+
+# ------------------------------------------------ full benchmark sizes ---
+
+def test_poisson3d_128_full_size_closed_form(cuda):
+    """The 128^3 7-point operator (n = 2,097,152): A*1 equals the count of
+    missing neighbours, exactly; a random x matches the oracle."""
+    import torch
+
+    side = 128
+    rp, ci, v = matgen.poisson3d(side)
+    n = side ** 3
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=16)
+    y = A.spmv(torch.ones(n, dtype=torch.float64, device="cuda")).cpu().numpy()
+    i = np.arange(side)
+    edge = (i == 0).astype(int) + (i == side - 1)
+    missing = edge[:, None, None] + edge[None, :, None] + edge[None, None, :]
+    np.testing.assert_array_equal(y, missing.ravel().astype(float))
+    x = np.random.default_rng(11).uniform(-1, 1, n)
+    assert_spmv_close(A.spmv(to_dev(x)).cpu().numpy(), O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
+
+
+def test_powerlaw_4m_full_size(cuda):
+    """The irregular power-law matrix at n = 4,000,000 against the oracle."""
+    n = 4_000_000
+    rp, ci, v = matgen.powerlaw(n, seed=2024)
+    x = np.random.default_rng(7).uniform(-1, 1, n)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=16)
+    y = A.spmv(to_dev(x)).cpu().numpy()
+    assert_spmv_close(y, O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
