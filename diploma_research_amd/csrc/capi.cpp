@@ -85,9 +85,9 @@ dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out) {
     DIM_HIP_TRY(dev_alloc(h.get(), tp, T * kMaxK));
     s.head = hp;
     s.tail = tp;
-    DIM_HIP_TRY(dev_alloc(h.get(), s.dot_tile, T * kMaxK));
-    s.fix_blocks = static_cast<uint32_t>((T + 255) / 256);
-    DIM_HIP_TRY(dev_alloc(h.get(), s.dot_blk, static_cast<size_t>(s.fix_blocks) * kMaxK));
+    DIM_HIP_TRY(dev_alloc(h.get(), s.chunk_flags, T));
+    DIM_HIP_TRY(dev_alloc(h.get(), s.dot_chunk, T * kMaxK));
+    DIM_HIP_TRY(dev_alloc(h.get(), s.dot_blk, ((T + 255) / 256) * kMaxK));
     h->sys_row = {0, f.rows};
     h->sys_tile = {0, f.tiles};
     h->sys_nnz_csr = {0, static_cast<uint32_t>(f.nnz)};

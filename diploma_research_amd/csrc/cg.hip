@@ -1,20 +1,672 @@
-// Conjugate-gradient solvers (filled in below).
+// Conjugate-gradient solvers on the CSR5 product.
+//
+// Textbook CG (thesis.tex:2133-2160) in the step order of the reference's
+// single-process driver (benchmarks/conjugate_gradient/dim_sp/main.cpp:
+// 102-127): As = A*s; alpha = r.r / s.As; x += alpha*s; r -= alpha*As;
+// r.r_new; stop test; beta = r.r_new / r.r; s = r + beta*s.
+//
+// One iteration is four kernels with every scalar kept on the device:
+//   1. CSR5 tiles  q = A p, with p[row]*q[row] folded into the epilogue;
+//   2. carry fix-up of rows that cross tiles, which also reduces the p.q
+//      partials of 256 tiles into one partial per block;
+//   3. x += alpha p, r -= alpha q and the r.r partials (alpha from the p.q
+//      partials, reduced redundantly and in a fixed order by every block);
+//   4. p = r + beta p, the stop test and the iteration counter.
+// There are no atomics, so iterates are bitwise reproducible. The r.r
+// partials are double-buffered by iteration parity, and a batch of an even
+// number of iterations is captured once in a hipGraph and replayed; every
+// kernel returns at once when the device-side `done` flag is set, so the
+// host only looks at the state between batches.
+//
+// k right-hand sides share one SpMM per iteration (independent CG per
+// column). The FP32-refined mode iterates in FP32 (values, vectors; dot
+// products accumulated in FP64) and replaces the residual in FP64 between
+// batches: x += x_corr, r = b - A x, both in FP64, and the stop test uses that
+// FP64 residual (reliable updates).
 #include "handle.h"
 
-extern "C" {
-dim_status dim_cg(const dim_csr5* A, const double*, double*, const dim_cg_opts*, dim_cg_result*, void*) {
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace dimgpu {
+
+namespace {
+
+constexpr int kVecThreads = 1024;
+constexpr int kMaxVecBlocks = 256;
+
+struct CgState {
+    int done; // 0 running, 1 converged, 2 breakdown, 3 iteration cap
+    int iterations;
+    int max_iters;
+    int k;
+    int col_done[kMaxK]; // 1 converged, 2 breakdown
+    int col_iters[kMaxK];
+    double tol2[kMaxK];  // (rtol * ||b_j||)^2
+    double bnorm[kMaxK];
+    double rr[kMaxK];     // r.r after the last completed iteration
+    double rr_true[kMaxK]; // FP64 residual norm^2 at the last residual replacement
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Sum of parts[i*k + j], i < n, computed by wave 0 in a fixed order and
+// broadcast through `out` (LDS). Every block obtains the same bits.
+template<int KMAX>
+__device__ void reduce_parts(const double* parts, int n, int k, double* out) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        for (int j = 0; j < k; ++j) {
+            double s = 0.0;
+            for (int i = lane; i < n; i += 64)
+                s += parts[static_cast<size_t>(i) * k + j];
+            s = wave_sum(s);
+            if (lane == 0)
+                out[j] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Block-wide sums of acc[j] (fixed order) written to parts[blockIdx.x*k + j].
+template<int KMAX>
+__device__ void block_store_sums(const double* acc, int k, double* parts) {
+    __shared__ double red[kVecThreads / 64][KMAX];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        const double s = wave_sum(acc[j]);
+        if (lane == 0)
+            red[w][j] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < static_cast<unsigned>(k)) {
+        double s = 0.0;
+        for (int i = 0; i < nw; ++i)
+            s += red[i][threadIdx.x];
+        parts[static_cast<size_t>(blockIdx.x) * k + threadIdx.x] = s;
+    }
+}
+
+// r = b - q (q = A x0), p = r; partials of r.r and b.b.
+template<typename V, int KMAX>
+__global__ __launch_bounds__(kVecThreads) void k_cg_init(size_t n, int k, const double* b, const double* q, V* r,
+                                                         V* p, double* rr_part, double* bb_part) {
+    double acc[KMAX], bb[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+        acc[j] = bb[j] = 0.0;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            if (j < k) {
+                const double bi = b[i * k + j];
+                const V ri = static_cast<V>(bi - q[i * k + j]);
+                r[i * k + j] = ri;
+                p[i * k + j] = ri;
+                acc[j] += static_cast<double>(ri) * static_cast<double>(ri);
+                bb[j] += bi * bi;
+            }
+        }
+    }
+    block_store_sums<KMAX>(acc, k, rr_part);
+    __syncthreads();
+    block_store_sums<KMAX>(bb, k, bb_part);
+}
+
+// Tolerances and the initial stop test (one block).
+template<int KMAX>
+__global__ void k_cg_init_state(CgState* st, int nparts, const double* rr_part, const double* bb_part, double rtol,
+                                double* hist) {
+    __shared__ double rr[KMAX], bb[KMAX];
+    const int k = st->k;
+    reduce_parts<KMAX>(rr_part, nparts, k, rr);
+    reduce_parts<KMAX>(bb_part, nparts, k, bb);
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int j = 0; j < k; ++j) {
+            const double bn = std::sqrt(bb[j]);
+            st->bnorm[j] = bn;
+            const double scale = bn > 0.0 ? bn : 1.0;
+            st->tol2[j] = (rtol * scale) * (rtol * scale);
+            st->rr[j] = rr[j];
+            st->rr_true[j] = rr[j];
+            st->col_done[j] = rr[j] <= st->tol2[j] ? 1 : 0;
+            st->col_iters[j] = 0;
+            all &= st->col_done[j] != 0;
+            if (hist)
+                hist[j] = std::sqrt(rr[j]) / scale;
+        }
+        st->iterations = 0;
+        st->done = all ? 1 : (st->max_iters == 0 ? 3 : 0);
+    }
+}
+
+// alpha = r.r / p.q; x += alpha p; r -= alpha q; r.r partials.
+template<typename V, int KMAX>
+__global__ __launch_bounds__(kVecThreads) void k_cg_update_xr(CgState* st, size_t n, int k, V* x, V* r, const V* p,
+                                                              const V* q, const double* pq_part, int n_pq,
+                                                              const double* rr_in, double* rr_out, int n_rr) {
+    if (st->done)
+        return;
+    __shared__ double pq[KMAX], rr[KMAX], alpha_s[KMAX];
+    reduce_parts<KMAX>(pq_part, n_pq, k, pq);
+    reduce_parts<KMAX>(rr_in, n_rr, k, rr);
+    if (threadIdx.x < static_cast<unsigned>(k)) {
+        const int j = threadIdx.x;
+        double a = rr[j] / pq[j];
+        const bool broke = !(pq[j] > 0.0) || !std::isfinite(a);
+        if (st->col_done[j] || broke)
+            a = 0.0;
+        alpha_s[j] = a;
+        if (broke && !st->col_done[j] && blockIdx.x == 0) {
+            st->col_done[j] = 2;
+            st->col_iters[j] = st->iterations;
+            st->done = 2;
+        }
+    }
+    __syncthreads();
+    V alpha[KMAX];
+    double acc[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        alpha[j] = static_cast<V>(j < k ? alpha_s[j] : 0.0);
+        acc[j] = 0.0;
+    }
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            if (j < k) {
+                const size_t e = i * k + j;
+                x[e] = x[e] + alpha[j] * p[e];
+                const V ri = r[e] - alpha[j] * q[e];
+                r[e] = ri;
+                acc[j] += static_cast<double>(ri) * static_cast<double>(ri);
+            }
+        }
+    }
+    block_store_sums<KMAX>(acc, k, rr_out);
+}
+
+// Stop test; beta = r.r_new / r.r; p = r + beta p.
+template<typename V, int KMAX>
+__global__ __launch_bounds__(kVecThreads) void k_cg_update_p(CgState* st, size_t n, int k, const V* r, V* p,
+                                                             const double* rr_new_part, const double* rr_old_part,
+                                                             int n_rr, double* hist, int hist_cap) {
+    if (st->done)
+        return;
+    __shared__ double rn[KMAX], ro[KMAX];
+    __shared__ int conv_s[KMAX];
+    __shared__ double beta_s[KMAX];
+    __shared__ int all_s;
+    reduce_parts<KMAX>(rr_new_part, n_rr, k, rn);
+    reduce_parts<KMAX>(rr_old_part, n_rr, k, ro);
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int j = 0; j < k; ++j) {
+            // a column frozen earlier (converged or broken down) keeps r, so
+            // its test repeats with the same numbers
+            const int c = (rn[j] <= st->tol2[j]) || (st->col_done[j] == 2);
+            conv_s[j] = c;
+            beta_s[j] = rn[j] / ro[j];
+            all &= c;
+        }
+        all_s = all;
+    }
+    __syncthreads();
+    V beta[KMAX];
+    bool conv[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        beta[j] = static_cast<V>(j < k ? beta_s[j] : 0.0);
+        conv[j] = j < k ? conv_s[j] != 0 : true;
+    }
+    if (!all_s) {
+        for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+             i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j) {
+                if (j < k && !conv[j]) {
+                    const size_t e = i * k + j;
+                    p[e] = r[e] + beta[j] * p[e];
+                }
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int it = st->iterations + 1;
+        st->iterations = it;
+        for (int j = 0; j < k; ++j) {
+            st->rr[j] = rn[j];
+            if (conv_s[j] && !st->col_done[j]) {
+                st->col_done[j] = 1;
+                st->col_iters[j] = it;
+            }
+            if (hist && it < hist_cap) {
+                const double scale = st->bnorm[j] > 0.0 ? st->bnorm[j] : 1.0;
+                hist[static_cast<size_t>(it) * k + j] = std::sqrt(rn[j]) / scale;
+            }
+        }
+        if (all_s)
+            st->done = 1;
+        else if (it >= st->max_iters)
+            st->done = 3;
+    }
+}
+
+// Reliable update, part 1: x += x_corr (FP64), x_corr = 0.
+template<int KMAX>
+__global__ __launch_bounds__(kVecThreads) void k_ru_fold(size_t n, int k, double* x, float* xc) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n * k;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        x[i] += static_cast<double>(xc[i]);
+        xc[i] = 0.0f;
+    }
+}
+
+// Reliable update, part 2: r64 = b - q64; r32 = float(r64); partials of the
+// FP32 residual's r.r (iteration scalar) and of the FP64 residual (stop test).
+template<int KMAX>
+__global__ __launch_bounds__(kVecThreads) void k_ru_residual(size_t n, int k, const double* b, const double* q,
+                                                             float* r32, double* rr_part, double* rt_part) {
+    double acc[KMAX], tr[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+        acc[j] = tr[j] = 0.0;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            if (j < k) {
+                const double r = b[i * k + j] - q[i * k + j];
+                const float rs = static_cast<float>(r);
+                r32[i * k + j] = rs;
+                acc[j] += static_cast<double>(rs) * static_cast<double>(rs);
+                tr[j] += r * r;
+            }
+        }
+    }
+    block_store_sums<KMAX>(acc, k, rr_part);
+    __syncthreads();
+    block_store_sums<KMAX>(tr, k, rt_part);
+}
+
+// Reliable update, part 3 (one block): the FP64 stop test decides; columns
+// whose FP64 residual is not yet small enough are resumed.
+template<int KMAX>
+__global__ void k_ru_state(CgState* st, int nparts, const double* rr_part, const double* rt_part) {
+    __shared__ double rr[KMAX], rt[KMAX];
+    const int k = st->k;
+    reduce_parts<KMAX>(rr_part, nparts, k, rr);
+    reduce_parts<KMAX>(rt_part, nparts, k, rt);
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int j = 0; j < k; ++j) {
+            st->rr[j] = rr[j];
+            st->rr_true[j] = rt[j];
+            if (st->col_done[j] != 2) {
+                const int c = rt[j] <= st->tol2[j];
+                if (c && st->col_done[j] != 1)
+                    st->col_iters[j] = st->iterations;
+                st->col_done[j] = c;
+            }
+            all &= st->col_done[j] != 0;
+        }
+        if (st->done != 2)
+            st->done = all ? 1 : (st->iterations >= st->max_iters ? 3 : 0);
+    }
+}
+
+// ||b - q||^2 partials (FP64) for the final true residual.
+template<int KMAX>
+__global__ __launch_bounds__(kVecThreads) void k_resid_parts(size_t n, int k, const double* b, const double* q,
+                                                             double* parts) {
+    double acc[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+        acc[j] = 0.0;
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j)
+            if (j < k) {
+                const double d = b[i * k + j] - q[i * k + j];
+                acc[j] += d * d;
+            }
+    }
+    block_store_sums<KMAX>(acc, k, parts);
+}
+
+template<typename V>
+__global__ void k_zero(V* p, size_t n) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        p[i] = V(0);
+}
+
+// ------------------------------------------------------------ host side --
+
+struct DevBuf {
+    std::vector<void*> ptrs;
+    ~DevBuf() {
+        for (void* p : ptrs)
+            (void)hipFree(p);
+    }
+    template<typename T>
+    hipError_t alloc(T*& p, size_t count) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess)
+            ptrs.push_back(q);
+        p = static_cast<T*>(q);
+        return e;
+    }
+};
+
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    ~Graph() {
+        if (ge)
+            (void)hipGraphExecDestroy(ge);
+        if (g)
+            (void)hipGraphDestroy(g);
+    }
+};
+
+struct PinnedState {
+    CgState* h = nullptr;
+    PinnedState() { (void)hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(CgState), hipHostMallocDefault); }
+    ~PinnedState() {
+        if (h)
+            (void)hipHostFree(h);
+    }
+};
+
+uint32_t vec_blocks(size_t n) {
+    return static_cast<uint32_t>(std::max<size_t>(1, std::min<size_t>(kMaxVecBlocks, (n + 4 * kVecThreads - 1) /
+                                                                                          (4 * kVecThreads))));
+}
+
+template<int KMAX>
+struct Launcher {
+    // one CG iteration in precision V (f32 picks the FP32 matrix values)
+    template<typename V>
+    static void iteration(dim_csr5* A, CgState* st, size_t n, int k, V* x, V* r, V* p, V* q, double* rr_a,
+                          double* rr_b, uint32_t nvb, double* hist, int hist_cap, hipStream_t s) {
+        constexpr bool f32 = std::is_same<V, float>::value;
+        uint32_t n_pq = 1;
+        (void)launch_spmv(A->f, A->s, f32, k, p, q, 0, A->f.tiles, 0, 0, A->f.n_empty, true, &st->done, s, &n_pq);
+        hipLaunchKernelGGL((k_cg_update_xr<V, KMAX>), dim3(nvb), dim3(kVecThreads), 0, s, st, n, k, x, r,
+                           static_cast<const V*>(p), static_cast<const V*>(q), A->s.dot_blk, static_cast<int>(n_pq), rr_a, rr_b,
+                           static_cast<int>(nvb));
+        hipLaunchKernelGGL((k_cg_update_p<V, KMAX>), dim3(nvb), dim3(kVecThreads), 0, s, st, n, k,
+                           static_cast<const V*>(r), p, rr_b, rr_a, static_cast<int>(nvb), hist, hist_cap);
+    }
+};
+
+struct SolveOut {
+    std::vector<CgState> unused;
+};
+
+template<int KMAX>
+dim_status solve(dim_csr5* A, int k, const double* b, double* x, const dim_cg_opts& o, dim_cg_result* res,
+                 hipStream_t stream) {
+    const size_t n = A->f.rows;
+    const bool mixed = o.precision == DIM_CG_F32_REFINED;
+    if (mixed && !A->f.vals32)
+        return fail(DIM_INVALID_ARGUMENT, "FP32-refined CG needs a matrix built with DIM_CSR5_KEEP_F32");
+    const uint32_t nvb = vec_blocks(n);
+    const uint32_t check = o.check_every ? std::max<uint32_t>(2, (o.check_every + 1) & ~1u) : 16;
+    const double delta = o.refine_delta > 0.0 ? o.refine_delta : 0.1;
+
+    DevBuf buf;
+    CgState* st = nullptr;
+    double *q64 = nullptr, *r64 = nullptr, *p64 = nullptr, *rr0 = nullptr, *rr1 = nullptr, *bb = nullptr,
+           *rt = nullptr, *hist = nullptr;
+    float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+    DIM_HIP_TRY(buf.alloc(st, 1));
+    DIM_HIP_TRY(buf.alloc(q64, n * k));
+    DIM_HIP_TRY(buf.alloc(rr0, size_t{nvb} * kMaxK));
+    DIM_HIP_TRY(buf.alloc(rr1, size_t{nvb} * kMaxK));
+    DIM_HIP_TRY(buf.alloc(bb, size_t{nvb} * kMaxK));
+    DIM_HIP_TRY(buf.alloc(rt, size_t{nvb} * kMaxK));
+    const int hist_cap = o.history ? static_cast<int>(o.max_iters) + 1 : 0;
+    if (hist_cap)
+        DIM_HIP_TRY(buf.alloc(hist, static_cast<size_t>(hist_cap) * k));
+    if (mixed) {
+        DIM_HIP_TRY(buf.alloc(r32, n * k));
+        DIM_HIP_TRY(buf.alloc(p32, n * k));
+        DIM_HIP_TRY(buf.alloc(q32, n * k));
+        DIM_HIP_TRY(buf.alloc(x32, n * k));
+    } else {
+        DIM_HIP_TRY(buf.alloc(r64, n * k));
+        DIM_HIP_TRY(buf.alloc(p64, n * k));
+    }
+    PinnedState hs;
+    if (!hs.h)
+        return fail(DIM_OUT_OF_MEMORY, "pinned host state");
+
+    // capture `check` iterations (even) once
+    Graph graph;
+    hipStream_t cap = nullptr;
+    DIM_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    bool use_graph = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    auto enqueue_batch = [&](hipStream_t s) {
+        for (uint32_t it = 0; it < check; it += 2) {
+            if (mixed) {
+                Launcher<KMAX>::template iteration<float>(A, st, n, k, x32, r32, p32, q32, rr0, rr1, nvb, hist,
+                                                          hist_cap, s);
+                Launcher<KMAX>::template iteration<float>(A, st, n, k, x32, r32, p32, q32, rr1, rr0, nvb, hist,
+                                                          hist_cap, s);
+            } else {
+                Launcher<KMAX>::template iteration<double>(A, st, n, k, x, r64, p64, q64, rr0, rr1, nvb, hist,
+                                                           hist_cap, s);
+                Launcher<KMAX>::template iteration<double>(A, st, n, k, x, r64, p64, q64, rr1, rr0, nvb, hist,
+                                                           hist_cap, s);
+            }
+        }
+    };
+    if (use_graph) {
+        enqueue_batch(cap);
+        use_graph = hipStreamEndCapture(cap, &graph.g) == hipSuccess && graph.g &&
+                    hipGraphInstantiate(&graph.ge, graph.g, nullptr, nullptr, 0) == hipSuccess;
+        (void)hipGetLastError();
+    }
+    (void)hipStreamDestroy(cap);
+
+    hipEvent_t e0, e1;
+    DIM_HIP_TRY(hipEventCreate(&e0));
+    DIM_HIP_TRY(hipEventCreate(&e1));
+    DIM_HIP_TRY(hipEventRecord(e0, stream));
+
+    CgState init{};
+    init.max_iters = static_cast<int>(o.max_iters);
+    init.k = k;
+    DIM_HIP_TRY(hipMemcpyAsync(st, &init, sizeof(CgState), hipMemcpyHostToDevice, stream));
+    // q = A x0 (FP64), r = b - q, p = r
+    DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false, nullptr, stream));
+    if (mixed) {
+        hipLaunchKernelGGL((k_cg_init<float, KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, b, q64, r32, p32,
+                           rr0, bb);
+        hipLaunchKernelGGL((k_zero<float>), dim3(nvb), dim3(kVecThreads), 0, stream, x32, n * k);
+    } else {
+        hipLaunchKernelGGL((k_cg_init<double, KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, b, q64, r64, p64,
+                           rr0, bb);
+    }
+    hipLaunchKernelGGL((k_cg_init_state<KMAX>), dim3(1), dim3(64), 0, stream, st, static_cast<int>(nvb), rr0, bb,
+                       o.rtol, hist);
+    DIM_HIP_TRY(hipGetLastError());
+
+    uint32_t refinements = 0;
+    std::vector<double> rr_at_update(k, 0.0);
+    bool first = true;
+    dim_status status = DIM_SUCCESS;
+    for (;;) {
+        DIM_HIP_TRY(hipMemcpyAsync(hs.h, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+        DIM_HIP_TRY(hipStreamSynchronize(stream));
+        if (first) {
+            for (int j = 0; j < k; ++j)
+                rr_at_update[j] = hs.h->rr_true[j];
+            first = false;
+        }
+        if (mixed && hs.h->done != 2) {
+            // residual replacement when a column's FP32 residual fell by delta
+            // since the last one, when the FP32 residual says converged, or at
+            // the iteration cap
+            bool need = hs.h->done == 1 || hs.h->done == 3;
+            for (int j = 0; j < k && !need; ++j)
+                need = !hs.h->col_done[j] && hs.h->rr[j] < delta * delta * rr_at_update[j];
+            if (need && hs.h->iterations > 0) {
+                ++refinements;
+                hipLaunchKernelGGL((k_ru_fold<KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, x, x32);
+                DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false,
+                                        nullptr, stream));
+                hipLaunchKernelGGL((k_ru_residual<KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, b, q64, r32,
+                                   rr0, rt);
+                hipLaunchKernelGGL((k_ru_state<KMAX>), dim3(1), dim3(64), 0, stream, st, static_cast<int>(nvb), rr0,
+                                   rt);
+                DIM_HIP_TRY(hipGetLastError());
+                DIM_HIP_TRY(hipMemcpyAsync(hs.h, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+                DIM_HIP_TRY(hipStreamSynchronize(stream));
+                for (int j = 0; j < k; ++j)
+                    rr_at_update[j] = hs.h->rr[j];
+            }
+        }
+        if (hs.h->done)
+            break;
+        if (use_graph) {
+            DIM_HIP_TRY(hipGraphLaunch(graph.ge, stream));
+        } else {
+            enqueue_batch(stream);
+            DIM_HIP_TRY(hipGetLastError());
+        }
+    }
+    DIM_HIP_TRY(hipEventRecord(e1, stream));
+    DIM_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    // true residual ||b - A x|| in FP64
+    DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false, nullptr, stream));
+    hipLaunchKernelGGL((k_resid_parts<KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, b, q64, rt);
+    DIM_HIP_TRY(hipGetLastError());
+    std::vector<double> parts(size_t{nvb} * k);
+    DIM_HIP_TRY(hipMemcpyAsync(parts.data(), rt, parts.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    DIM_HIP_TRY(hipStreamSynchronize(stream));
+    if (hist_cap) {
+        std::vector<double> hh(static_cast<size_t>(hist_cap) * k);
+        DIM_HIP_TRY(hipMemcpy(hh.data(), hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // column-major per column j: opts.history holds column 0 (single RHS use)
+        for (int i = 0; i < hist_cap; ++i)
+            o.history[i] = hh[static_cast<size_t>(i) * k];
+    }
+
+    const CgState& S = *hs.h;
+    for (int j = 0; j < k; ++j) {
+        double tr = 0.0;
+        for (uint32_t i = 0; i < nvb; ++i)
+            tr += parts[size_t{i} * k + j];
+        const double scale = S.bnorm[j] > 0.0 ? S.bnorm[j] : 1.0;
+        dim_cg_result& R = res[j];
+        R.converged = S.col_done[j] == 1;
+        R.iterations = static_cast<uint32_t>(S.col_done[j] ? S.col_iters[j] : S.iterations);
+        R.relres = std::sqrt(mixed ? S.rr_true[j] : S.rr[j]) / scale;
+        R.relres_true = std::sqrt(tr) / scale;
+        R.refinements = refinements;
+        R.seconds = ms * 1e-3;
+        if (S.col_done[j] == 2)
+            status = DIM_BREAKDOWN;
+        else if (!R.converged && status == DIM_SUCCESS)
+            status = DIM_NOT_CONVERGED;
+    }
+    if (status == DIM_BREAKDOWN)
+        set_error("CG breakdown: p'Ap <= 0 or not finite (matrix not SPD?)");
+    else if (status == DIM_NOT_CONVERGED)
+        set_error("CG reached max_iters above the tolerance");
+    return status;
+}
+
+dim_status run(dim_csr5* A, int k, const double* b, double* x, const dim_cg_opts* opts, dim_cg_result* res,
+               hipStream_t s) {
+    if (k == 1)
+        return solve<1>(A, k, b, x, *opts, res, s);
+    if (k <= 2)
+        return solve<2>(A, k, b, x, *opts, res, s);
+    if (k <= 4)
+        return solve<4>(A, k, b, x, *opts, res, s);
+    if (k <= 8)
+        return solve<8>(A, k, b, x, *opts, res, s);
+    return solve<16>(A, k, b, x, *opts, res, s);
+}
+
+dim_status check_cg_args(const dim_csr5* A, uint32_t k, const void* b, const void* x, const dim_cg_opts* opts,
+                         const dim_cg_result* res) {
     if (!A)
-        return dimgpu::fail(DIM_INVALID_ARGUMENT, "matrix handle is null");
-    return dimgpu::fail(DIM_UNSUPPORTED, "not built yet");
+        return fail(DIM_INVALID_ARGUMENT, "matrix handle is null");
+    if (!opts || !res)
+        return fail(DIM_INVALID_ARGUMENT, "opts and res are required");
+    if (A->f.rows != A->f.cols)
+        return fail(DIM_INVALID_ARGUMENT, "CG needs a square matrix");
+    if (k < 1 || k > static_cast<uint32_t>(kMaxK))
+        return fail(DIM_INVALID_ARGUMENT, "k must be in 1..16");
+    if (A->f.rows && (!b || !x))
+        return fail(DIM_INVALID_ARGUMENT, "null vector");
+    if (!(opts->rtol >= 0.0))
+        return fail(DIM_INVALID_ARGUMENT, "rtol must be >= 0");
+    if (opts->precision != DIM_CG_F64 && opts->precision != DIM_CG_F32_REFINED)
+        return fail(DIM_INVALID_ARGUMENT, "unknown precision");
+    if (opts->history && k != 1)
+        return fail(DIM_INVALID_ARGUMENT, "history is recorded for single right-hand sides only");
+    if (A->n_systems != 1)
+        return fail(DIM_INVALID_ARGUMENT, "batched matrices are solved with dim_cg_batched");
+    return DIM_SUCCESS;
 }
-dim_status dim_cg_multi(const dim_csr5*, uint32_t, const double*, double*, const dim_cg_opts*, dim_cg_result*, void*) {
-    return dimgpu::fail(DIM_UNSUPPORTED, "not built yet");
+
+} // namespace
+} // namespace dimgpu
+
+using namespace dimgpu;
+
+extern "C" {
+
+dim_status dim_cg(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts, dim_cg_result* res,
+                  void* stream) {
+    if (dim_status st = check_cg_args(A, 1, b, x, opts, res))
+        return st;
+    return run(const_cast<dim_csr5*>(A), 1, b, x, opts, res, static_cast<hipStream_t>(stream));
 }
+
+dim_status dim_cg_multi(const dim_csr5* A, uint32_t k, const double* B, double* X, const dim_cg_opts* opts,
+                        dim_cg_result* res, void* stream) {
+    if (dim_status st = check_cg_args(A, k, B, X, opts, res))
+        return st;
+    return run(const_cast<dim_csr5*>(A), static_cast<int>(k), B, X, opts, res, static_cast<hipStream_t>(stream));
+}
+
 dim_status dim_csr5_from_csr_batched(uint32_t, const uint32_t*, const uint64_t*, const uint32_t* const*,
                                      const uint32_t* const*, const double* const*, const dim_csr5_opts*, dim_csr5**) {
-    return dimgpu::fail(DIM_UNSUPPORTED, "not built yet");
+    return fail(DIM_UNSUPPORTED, "not built yet");
 }
-dim_status dim_cg_batched(const dim_csr5*, const double*, double*, const dim_cg_opts*, dim_cg_result*, void*) {
-    return dimgpu::fail(DIM_UNSUPPORTED, "not built yet");
+
+dim_status dim_cg_batched(const dim_csr5* A, const double*, double*, const dim_cg_opts*, dim_cg_result*, void*) {
+    if (!A)
+        return fail(DIM_INVALID_ARGUMENT, "matrix handle is null");
+    return fail(DIM_UNSUPPORTED, "not built yet");
 }
-}
+
+} // extern "C"

@@ -29,25 +29,24 @@ struct DevFormat {
 // csr5/calibrator.h:14-31). Sized for up to kMaxK right-hand sides.
 constexpr int kMaxK = 16;
 struct Scratch {
-    void* head = nullptr;     // tiles * kMaxK * 8 bytes
-    void* tail = nullptr;     // tiles * kMaxK * 8 bytes
-    double* dot_tile = nullptr; // tiles * kMaxK
-    double* dot_blk = nullptr;  // ceil(tiles/256) * kMaxK
-    uint32_t fix_blocks = 0;
+    void* head = nullptr;          // chunks * kMaxK * 8 bytes (chunks <= tiles)
+    void* tail = nullptr;          // chunks * kMaxK * 8 bytes
+    uint8_t* chunk_flags = nullptr; // chunks
+    double* dot_chunk = nullptr;   // chunks * kMaxK
+    double* dot_blk = nullptr;     // ceil(chunks/256) * kMaxK
 };
-
-// Which reduction the tile kernels fold into their epilogue.
-enum class Epilogue : int { none = 0, dot_xy = 1 };
 
 // y = A x over tiles [tile_begin, tile_end). x and y are n x k, row-major.
 // row_base: y row index offset (0 for the whole matrix; first row of a block).
 // When `dot` is set (square matrices, whole matrix only) the per-column
-// products x[row]*y[row] are reduced into scratch.dot_blk.
+// products x[row]*y[row] are reduced into scratch.dot_blk, one partial per
+// fix-up block; *dot_blocks (if given) receives that count.
 // `done` (may be null): a device flag; when *done != 0 the kernels return at
-// once (used by batched solver launches).
+// once (used by solver launch sequences).
 hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y,
                        uint32_t tile_begin, uint32_t tile_end, uint32_t row_base, uint32_t empty_begin,
-                       uint32_t empty_end, bool dot, const int* done, hipStream_t stream);
+                       uint32_t empty_end, bool dot, const int* done, hipStream_t stream,
+                       uint32_t* dot_blocks = nullptr);
 
 // Same as launch_spmv over the whole matrix with events recorded before the
 // tile kernel, between the two kernels and after the fix-up kernel.

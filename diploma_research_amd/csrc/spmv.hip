@@ -126,98 +126,92 @@ TileParams make_tile_params(const DevFormat& A, bool f32) {
 
 using namespace detail;
 
-hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
-                       uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, bool dot,
-                       const int* done, hipStream_t stream) {
-    TileParams tp = make_tile_params(A, f32);
+int detail::device_cus() {
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return 256;
+    if (!cus[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
+namespace {
+
+void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
+                 uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, const int* done,
+                 TileParams& tp, FixParams& fp) {
+    tp = make_tile_params(A, f32);
     tp.x = x;
     tp.y = y;
     tp.head = S.head;
     tp.tail = S.tail;
-    tp.dot_tile = S.dot_tile;
+    tp.chunk_flags = S.chunk_flags;
+    tp.dot_chunk = S.dot_chunk;
     tp.done = done;
     tp.tile_begin = tile_begin;
     tp.tile_end = tile_end;
     tp.row_base = row_base;
     tp.k = k;
-
-    FixParams fp{};
+    fp = FixParams{};
     fp.tile_ptr = A.tile_ptr;
-    fp.tile_flags = A.tile_flags;
     fp.empty_rows = A.empty_rows;
+    fp.chunk_flags = S.chunk_flags;
     fp.x = x;
     fp.y = y;
     fp.head = S.head;
     fp.tail = S.tail;
-    fp.dot_tile = S.dot_tile;
+    fp.dot_chunk = S.dot_chunk;
     fp.dot_blk = S.dot_blk;
     fp.done = done;
     fp.tile_begin = tile_begin;
-    fp.tile_end = tile_end;
     fp.row_base = row_base;
     fp.empty_begin = empty_begin;
     fp.empty_end = empty_end;
     fp.k = k;
+}
 
-    const uint32_t ntiles = tile_end - tile_begin;
-    const uint32_t tgrid = (ntiles + 3) / 4;
-    const uint32_t fgrid = ntiles ? (ntiles + 255) / 256 : 1;
-    if (k == 1) {
-        if (f32)
-            launch_pair<float, 1>(A, tp, fp, tgrid, fgrid, dot, stream);
-        else
-            launch_pair<double, 1>(A, tp, fp, tgrid, fgrid, dot, stream);
-    } else if (k <= 2) {
-        launch_pair_k2(f32, A, tp, fp, tgrid, fgrid, dot, stream);
-    } else if (k <= 4) {
-        launch_pair_k4(f32, A, tp, fp, tgrid, fgrid, dot, stream);
-    } else if (k <= 8) {
-        launch_pair_k8(f32, A, tp, fp, tgrid, fgrid, dot, stream);
-    } else {
-        launch_pair_k16(f32, A, tp, fp, tgrid, fgrid, dot, stream);
-    }
+} // namespace
+
+hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
+                       uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, bool dot,
+                       const int* done, hipStream_t stream, uint32_t* dot_blocks) {
+    TileParams tp;
+    FixParams fp;
+    fill_params(A, S, f32, k, x, y, tile_begin, tile_end, row_base, empty_begin, empty_end, done, tp, fp);
+    uint32_t nb;
+    if (k == 1)
+        nb = f32 ? launch_pair<float, 1>(A, tp, fp, dot, stream, nullptr)
+                 : launch_pair<double, 1>(A, tp, fp, dot, stream, nullptr);
+    else if (k <= 2)
+        nb = launch_pair_k2(f32, A, tp, fp, dot, stream);
+    else if (k <= 4)
+        nb = launch_pair_k4(f32, A, tp, fp, dot, stream);
+    else if (k <= 8)
+        nb = launch_pair_k8(f32, A, tp, fp, dot, stream);
+    else
+        nb = launch_pair_k16(f32, A, tp, fp, dot, stream);
+    if (dot_blocks)
+        *dot_blocks = nb;
     return hipGetLastError();
 }
 
 hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const void* x, void* y, hipEvent_t e0,
                               hipEvent_t e1, hipEvent_t e2, hipStream_t stream) {
-    TileParams tp = make_tile_params(A, f32);
-    tp.x = x;
-    tp.y = y;
-    tp.head = S.head;
-    tp.tail = S.tail;
-    tp.dot_tile = S.dot_tile;
-    FixParams fp{};
-    fp.tile_ptr = A.tile_ptr;
-    fp.tile_flags = A.tile_flags;
-    fp.empty_rows = A.empty_rows;
-    fp.x = x;
-    fp.y = y;
-    fp.head = S.head;
-    fp.tail = S.tail;
-    fp.dot_tile = S.dot_tile;
-    fp.dot_blk = S.dot_blk;
-    fp.tile_end = A.tiles;
-    fp.empty_end = A.n_empty;
-    fp.k = 1;
-    const uint32_t tgrid = (A.tiles + 3) / 4;
-    const uint32_t fgrid = A.tiles ? (A.tiles + 255) / 256 : 1;
+    TileParams tp;
+    FixParams fp;
+    fill_params(A, S, f32, 1, x, y, 0, A.tiles, 0, 0, A.n_empty, nullptr, tp, fp);
     hipError_t e = hipEventRecord(e0, stream);
     if (e != hipSuccess)
         return e;
-    if (tgrid) {
-        if (f32)
-            launch_by_sigma<float, 1, false>(A.sigma, tp, tgrid, stream);
-        else
-            launch_by_sigma<double, 1, false>(A.sigma, tp, tgrid, stream);
-    }
-    e = hipEventRecord(e1, stream);
-    if (e != hipSuccess)
-        return e;
     if (f32)
-        hipLaunchKernelGGL((k_csr5_fixup<float, 1, false>), dim3(fgrid), dim3(256), 0, stream, fp);
+        launch_pair<float, 1>(A, tp, fp, false, stream, e1);
     else
-        hipLaunchKernelGGL((k_csr5_fixup<double, 1, false>), dim3(fgrid), dim3(256), 0, stream, fp);
+        launch_pair<double, 1>(A, tp, fp, false, stream, e1);
     e = hipEventRecord(e2, stream);
     if (e != hipSuccess)
         return e;

@@ -1,23 +1,27 @@
 // CSR5 SpMV / SpMM kernels for gfx950 (CDNA4, wave64).
 //
-// One wavefront owns one tile of 64 columns x sigma steps. The reference does
-// the same work with four AVX2 lanes per tile and OpenMP threads over tile
-// chunks (include/dim/mat/storage_formats/csr5.h:619-748); here the 64 tile
-// columns are the 64 lanes, so:
-//   * step r of the tile is one coalesced load of 64 values and 64 column
+// The reference runs CSR5 with four AVX2 lanes per tile and OpenMP threads
+// over contiguous tile chunks (include/dim/mat/storage_formats/csr5.h:
+// 608-748). Here a tile is 64 columns wide, one column per lane of a
+// wavefront, and every wavefront walks one contiguous chunk of tiles:
+//   * step r of a tile is one coalesced load of 64 values and 64 column
 //     indices (the per-tile transpose of csr5.h:463-493 puts them side by side);
-//   * the lane's bit flags are decoded in registers, the per-column y_offset is
+//   * the lane's bit flags are decoded in registers: the per-column y_offset is
 //     a wavefront prefix sum of popcounts and scansum_offset is read off a
 //     ballot of the lanes that start a row (csr5.h:74-97);
 //   * rows that cross lanes are closed with a segmented wavefront scan of the
 //     lanes' leading partial sums, which replaces fast_segmented_sum /
-//     compute_last_sum (csr5.h:200-235) and keeps each sum in left-to-right
-//     order instead of differencing prefix sums;
-//   * rows that cross tiles leave their partial sums in per-tile head/tail
-//     slots, and a second kernel adds the chains up in tile order and writes
-//     those rows once. This replaces the reference's per-thread calibrator
-//     (csr5.h:526-538, 750-754) without atomics, so results are bitwise
-//     reproducible from run to run.
+//     compute_last_sum (csr5.h:200-235) and adds each row left to right
+//     instead of differencing prefix sums;
+//   * a row that crosses from one tile into the next tile of the same chunk is
+//     carried in registers; only rows that cross a chunk boundary leave
+//     partial sums in per-chunk head/tail slots, and a small second kernel
+//     adds those chains up in chunk order and writes each such row once. This
+//     replaces the reference's per-thread calibrator (csr5.h:526-538,
+//     750-754) without atomics, so results are bitwise reproducible.
+// Chunks are sized so that all wavefronts of the launch are resident at once
+// (one chunk per wavefront): a launch of one short wavefront per tile is
+// limited by the rate at which workgroups are dispatched.
 #pragma once
 
 #include "dim_internal.h"
@@ -42,7 +46,7 @@ struct DescWord<16> {
 };
 
 // Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD works on
-// one contiguous eighth of the tiles and its L2 sees neighbouring x entries.
+// one contiguous eighth of the chunks and its L2 sees neighbouring x entries.
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
     constexpr uint32_t X = 8;
     const uint32_t q = G / X, r = G % X, x = b % X, i = b / X;
@@ -71,6 +75,11 @@ __device__ __forceinline__ V ldg_stream(const V* p) {
     return __builtin_nontemporal_load(p);
 }
 
+// per-chunk flags written by the tile kernel
+constexpr uint8_t kChunkStarts = 1; // the chunk's first element starts a row
+constexpr uint8_t kChunkHead = 2;   // head slot holds the closing part of a row from earlier chunks
+constexpr uint8_t kChunkTail = 4;   // tail slot holds the open part of a row that continues
+
 struct TileParams {
     const uint32_t* tile_ptr;
     const void* bit_flag;
@@ -83,10 +92,12 @@ struct TileParams {
     void* y;
     void* head;
     void* tail;
-    double* dot_tile;
+    uint8_t* chunk_flags;
+    double* dot_chunk;
     const int* done;
     uint64_t nnz;
     uint32_t tile_begin, tile_end, row_base, last_tile;
+    uint32_t tpw, nchunks;
     int k;
 };
 
@@ -95,242 +106,296 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     if (p.done && *p.done)
         return;
     const int lane = threadIdx.x & 63;
-    const uint32_t t = p.tile_begin + xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (t >= p.tile_end)
+    const uint32_t c = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (c >= p.nchunks)
         return; // wave-uniform
+    const uint32_t t0 = p.tile_begin + c * p.tpw;
+    const uint32_t t1 = min(t0 + p.tpw, p.tile_end);
 
     using W = typename DescWord<SIGMA>::type;
     const int k = KMAX == 1 ? 1 : p.k;
     const V* __restrict__ x = static_cast<const V*>(p.x);
     V* __restrict__ y = static_cast<V*>(p.y);
-    V* __restrict__ head = static_cast<V*>(p.head);
-    V* __restrict__ tail = static_cast<V*>(p.tail);
 
-    const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
-    const V* vals = static_cast<const V*>(p.vals) + base + lane;
-    const uint32_t* cols = p.col_idx + base + lane;
-
-    // Single-vector products keep the whole lane column in registers so that
-    // all 2*sigma loads are in flight at once; multi-RHS products stream the
-    // column step by step (their k-wide x rows dominate the register budget).
-    constexpr int NREG = KMAX == 1 ? SIGMA : 1;
-    V v[NREG];
-    uint32_t c[NREG];
-    if constexpr (KMAX == 1) {
-#pragma unroll
-        for (int r = 0; r < SIGMA; ++r) {
-            v[r] = ldg_stream(vals + r * 64);
-            c[r] = ldg_stream(cols + r * 64);
-        }
-    }
-
-    const uint32_t bits = static_cast<uint32_t>(static_cast<const W*>(p.bit_flag)[static_cast<size_t>(t) * 64 + lane]) |
-                          (lane == 0 ? 1u : 0u);
-    const uint8_t tf = p.tile_flags[t];
-    const bool starts_row = tf & kStartsRow;
-    const uint32_t tpw = p.tile_ptr[t];
-    const bool dirty = (tpw & kDirty) != 0;
-    const uint32_t row0 = tpw & ~kDirty;
-    const uint32_t* tdo = dirty ? p.tdo + p.tdo_ptr[t] : nullptr;
-
-    // padded elements of the last tile contribute exactly zero
-    uint32_t pad = 0;
-    if (t == p.last_tile) {
-#pragma unroll
-        for (int r = 0; r < SIGMA; ++r)
-            if (base + static_cast<size_t>(r) * 64 + lane >= p.nnz)
-                pad |= 1u << r;
-    }
-
-    const int nb = __popc(bits);
-    const int incl = wave_incl_scan(nb, lane);
-    const int y_off = incl - nb;      // index of the first segment that starts in this lane
-    const int nseg = __shfl(incl, 63, 64);
-    const uint64_t started = __ballot(nb > 0);
-
-    V cur[KMAX], red[KMAX];
+    // The row left open by the previous tile of this chunk (its partial sum),
+    // and whether that row began before the chunk: its partial then goes to
+    // the chunk's head slot instead of y.
+    V carry[KMAX];
     double dacc[KMAX];
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
-        cur[j] = V(0);
-        red[j] = V(0);
+        carry[j] = V(0);
         dacc[j] = 0.0;
     }
+    const bool chunk_starts = p.tile_flags[t0] & kStartsRow;
+    bool carry_is_head = !chunk_starts;
+    uint8_t cflags = chunk_starts ? kChunkStarts : 0;
 
-    auto emit = [&](int s, const V* val) {
-        if (s == 0 && !starts_row) {
-            // the tile's first segment continues a row from an earlier tile
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                if (j < k)
-                    head[static_cast<size_t>(t) * k + j] = val[j];
-            return;
-        }
-        const uint32_t row = row0 + (dirty ? tdo[s] : static_cast<uint32_t>(s));
-        V* yr = y + static_cast<size_t>(row - p.row_base) * k;
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j)
-            if (j < k)
-                yr[j] = val[j];
-        if constexpr (DOT) {
-            const V* pr = x + static_cast<size_t>(row) * k;
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                if (j < k)
-                    dacc[j] += static_cast<double>(pr[j]) * static_cast<double>(val[j]);
-        }
-    };
+    for (uint32_t t = t0; t < t1; ++t) {
+        const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
+        const V* vals = static_cast<const V*>(p.vals) + base + lane;
+        const uint32_t* cols = p.col_idx + base + lane;
 
-    bool has = false;
-    int seg = y_off - 1;
-    auto step = [&](int r, V vr, uint32_t cr) {
-        if ((bits >> r) & 1u) {
-            if (has)
-                emit(seg, cur);
-            else
+        // Single-vector products keep the lane's whole column in registers so
+        // that all 2*sigma loads are in flight at once; multi-RHS products
+        // stream the column step by step (their k-wide x rows dominate).
+        constexpr int NREG = KMAX == 1 ? SIGMA : 1;
+        V v[NREG];
+        uint32_t cidx[NREG];
+        if constexpr (KMAX == 1) {
 #pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    red[j] = cur[j];
-            has = true;
-            ++seg;
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                cur[j] = V(0);
-        }
-        const V* xr = x + static_cast<size_t>(cr) * k;
-        const bool pz = (pad >> r) & 1u;
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-            if (j < k) {
-                V xv = xr[j];
-                if (pz)
-                    xv = V(0);
-                cur[j] = fma(vr, xv, cur[j]);
+            for (int r = 0; r < SIGMA; ++r) {
+                v[r] = ldg_stream(vals + r * 64);
+                cidx[r] = ldg_stream(cols + r * 64);
             }
         }
-    };
-    if constexpr (KMAX == 1) {
-#pragma unroll
-        for (int r = 0; r < SIGMA; ++r)
-            step(r, v[r], c[r]);
-    } else {
-#pragma unroll 1
-        for (int r = 0; r < SIGMA; ++r)
-            step(r, ldg_stream(vals + r * 64), ldg_stream(cols + r * 64));
-    }
 
-    V blue[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j) {
-        if (has) {
-            blue[j] = cur[j];
-        } else {
-            red[j] = cur[j];
-            blue[j] = V(0);
-        }
-    }
-
-    // Segmented inclusive scan of the lanes' leading (red) partials; a new
-    // segment begins after every lane in which a row starts.
-    int f = (lane == 0) ? 1 : static_cast<int>((started >> (lane - 1)) & 1ull);
-    V A[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j)
-        A[j] = red[j];
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int fup = __shfl_up(f, d, 64);
-        V up[KMAX];
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j)
-            up[j] = __shfl_up(A[j], d, 64);
-        if (lane >= d && !f) {
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                A[j] = up[j] + A[j];
-            f = fup;
-        }
-    }
-
-    // A lane in which a row starts closes the row left open by the previous
-    // such lane: its trailing (blue) partial plus the scanned red partials.
-    const uint64_t below = lane ? (started & ((1ull << lane) - 1ull)) : 0ull;
-    const int cprev = below ? 63 - __clzll(below) : 0;
-    V bc[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j)
-        bc[j] = __shfl(blue[j], cprev, 64);
-    if (lane > 0 && has) {
-        V tot[KMAX];
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j)
-            tot[j] = bc[j] + A[j];
-        emit(y_off - 1, tot);
-    }
-
-    // The tile's last segment.
-    const int L = 63 - __clzll(started);
-    V bl[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j)
-        bl[j] = __shfl(blue[j], L, 64);
-    if (lane == 63) {
-        V tv[KMAX];
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j)
-            tv[j] = (L == 63) ? blue[j] : bl[j] + A[j];
+        const uint32_t bits =
+          static_cast<uint32_t>(static_cast<const W*>(p.bit_flag)[static_cast<size_t>(t) * 64 + lane]) |
+          (lane == 0 ? 1u : 0u);
+        const uint8_t tf = p.tile_flags[t];
+        const bool starts_row = tf & kStartsRow;
+        const uint32_t tpw = p.tile_ptr[t];
+        const bool dirty = (tpw & kDirty) != 0;
+        const uint32_t row0 = tpw & ~kDirty;
+        const uint32_t* tdo = dirty ? p.tdo + p.tdo_ptr[t] : nullptr;
         const bool complete = (t + 1 == p.tile_end) || (p.tile_flags[t + 1] & kStartsRow);
-        if (nseg == 1 && !starts_row) {
+
+        // padded elements of the last tile contribute exactly zero
+        uint32_t pad = 0;
+        if (t == p.last_tile) {
 #pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                if (j < k) {
-                    head[static_cast<size_t>(t) * k + j] = tv[j];
-                    tail[static_cast<size_t>(t) * k + j] = tv[j];
-                }
-        } else if (complete) {
-            emit(nseg - 1, tv);
-        } else {
+            for (int r = 0; r < SIGMA; ++r)
+                if (base + static_cast<size_t>(lane) * SIGMA + r >= p.nnz)
+                    pad |= 1u << r;
+        }
+
+        const int nb = __popc(bits);
+        const int incl = wave_incl_scan(nb, lane);
+        const int y_off = incl - nb; // first segment that starts in this lane
+        const int nseg = __shfl(incl, 63, 64);
+        const uint64_t started = __ballot(nb > 0);
+
+        auto write_row = [&](uint32_t row, const V* val) {
+            V* yr = y + static_cast<size_t>(row - p.row_base) * k;
 #pragma unroll
             for (int j = 0; j < KMAX; ++j)
                 if (j < k)
-                    tail[static_cast<size_t>(t) * k + j] = tv[j];
+                    yr[j] = val[j];
+            if constexpr (DOT) {
+                const V* pr = x + static_cast<size_t>(row) * k;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    if (j < k)
+                        dacc[j] += static_cast<double>(pr[j]) * static_cast<double>(val[j]);
+            }
+        };
+        // segment 0 of a tile that does not start a row closes the carried row
+        auto close_carried = [&](const V* val) {
+            V tot[KMAX];
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j)
+                tot[j] = carry[j] + val[j];
+            if (carry_is_head) {
+                V* h = static_cast<V*>(p.head) + static_cast<size_t>(c) * k;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    if (j < k)
+                        h[j] = tot[j];
+            } else {
+                write_row(row0, tot);
+            }
+        };
+        auto emit = [&](int s, const V* val) {
+            if (s == 0 && !starts_row)
+                close_carried(val);
+            else
+                write_row(row0 + (dirty ? tdo[s] : static_cast<uint32_t>(s)), val);
+        };
+
+        V cur[KMAX], red[KMAX];
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            cur[j] = V(0);
+            red[j] = V(0);
+        }
+        bool has = false;
+        int seg = y_off - 1;
+        auto step = [&](int r, V vr, uint32_t cr) {
+            if ((bits >> r) & 1u) {
+                if (has)
+                    emit(seg, cur);
+                else
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j)
+                        red[j] = cur[j];
+                has = true;
+                ++seg;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    cur[j] = V(0);
+            }
+            const V* xr = x + static_cast<size_t>(cr) * k;
+            const bool pz = (pad >> r) & 1u;
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j) {
+                if (j < k) {
+                    V xv = xr[j];
+                    if (pz)
+                        xv = V(0);
+                    cur[j] = fma(vr, xv, cur[j]);
+                }
+            }
+        };
+        if constexpr (KMAX == 1) {
+#pragma unroll
+            for (int r = 0; r < SIGMA; ++r)
+                step(r, v[r], cidx[r]);
+        } else {
+#pragma unroll 1
+            for (int r = 0; r < SIGMA; ++r)
+                step(r, ldg_stream(vals + r * 64), ldg_stream(cols + r * 64));
+        }
+
+        V blue[KMAX];
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            if (has) {
+                blue[j] = cur[j];
+            } else {
+                red[j] = cur[j];
+                blue[j] = V(0);
+            }
+        }
+
+        // Segmented inclusive scan of the lanes' leading (red) partials; a new
+        // segment begins after every lane in which a row starts.
+        int f = (lane == 0) ? 1 : static_cast<int>((started >> (lane - 1)) & 1ull);
+        V A[KMAX];
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j)
+            A[j] = red[j];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int fup = __shfl_up(f, d, 64);
+            V up[KMAX];
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j)
+                up[j] = __shfl_up(A[j], d, 64);
+            if (lane >= d && !f) {
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    A[j] = up[j] + A[j];
+                f = fup;
+            }
+        }
+
+        // A lane in which a row starts closes the row left open by the
+        // previous such lane: its trailing (blue) partial plus the scanned reds.
+        const uint64_t below = lane ? (started & ((1ull << lane) - 1ull)) : 0ull;
+        const int cprev = below ? 63 - __clzll(below) : 0;
+        V bc[KMAX];
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j)
+            bc[j] = __shfl(blue[j], cprev, 64);
+        if (lane > 0 && has) {
+            V tot[KMAX];
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j)
+                tot[j] = bc[j] + A[j];
+            emit(y_off - 1, tot);
+        }
+
+        // The tile's last segment, known to every lane.
+        const int L = 63 - __clzll(started);
+        V tv[KMAX];
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            const V bl = __shfl(blue[j], L, 64);
+            const V a63 = __shfl(A[j], 63, 64);
+            const V b63 = __shfl(blue[j], 63, 64);
+            tv[j] = (L == 63) ? b63 : bl + a63;
+        }
+        if (nseg == 1 && !starts_row) {
+            // the whole tile continues the carried row
+            if (complete) {
+                if (lane == 0)
+                    close_carried(tv);
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    carry[j] = V(0);
+                carry_is_head = false;
+            } else {
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    carry[j] = carry[j] + tv[j];
+            }
+        } else {
+            if (!starts_row)
+                carry_is_head = false; // segment 0 closed the carried row above
+            if (complete) {
+                if (lane == 63)
+                    emit(nseg - 1, tv);
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    carry[j] = V(0);
+            } else {
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    carry[j] = tv[j];
+            }
+        }
+        if (t + 1 == t1 && !complete) {
+            // the chunk ends inside a row
+            cflags |= kChunkTail;
+            if (lane == 0) {
+                V* tl = static_cast<V*>(p.tail) + static_cast<size_t>(c) * k;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    if (j < k)
+                        tl[j] = carry[j];
+            }
         }
     }
-
+    if (!chunk_starts && !carry_is_head)
+        cflags |= kChunkHead;
+    if (lane == 0)
+        p.chunk_flags[c] = cflags;
     if constexpr (DOT) {
 #pragma unroll
         for (int j = 0; j < KMAX; ++j) {
             const double s = wave_sum(dacc[j]);
             if (lane == 0 && j < k)
-                p.dot_tile[static_cast<size_t>(t) * k + j] = s;
+                p.dot_chunk[static_cast<size_t>(c) * k + j] = s;
         }
     }
 }
 
 struct FixParams {
     const uint32_t* tile_ptr;
-    const uint8_t* tile_flags;
     const uint32_t* empty_rows;
+    const uint8_t* chunk_flags;
     const void* x;
     void* y;
     const void* head;
     const void* tail;
-    const double* dot_tile;
+    const double* dot_chunk;
     double* dot_blk;
     const int* done;
-    uint32_t tile_begin, tile_end, row_base, empty_begin, empty_end;
+    uint32_t tile_begin, row_base, empty_begin, empty_end;
+    uint32_t tpw, nchunks;
     int k;
 };
 
-// Rows that cross tile boundaries: the tile in which such a row ends adds the
-// chain tail(start) + tail(middle tiles...) + head(end) in tile order and
-// writes the row once. Also writes 0 to empty rows and, for CG, reduces the
-// per-tile dot partials of 256 tiles to one partial per block.
+// Rows that cross chunk boundaries: the chunk in which such a row ends adds
+// the chain tail(start chunk) + tail(chunks inside the row) + head(end chunk)
+// in chunk order and writes the row once. Also writes 0 to empty rows and,
+// for CG, reduces the per-chunk dot partials to one partial per block.
 template<typename V, int KMAX, bool DOT>
 __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
     if (p.done && *p.done)
         return;
-    const uint32_t t = p.tile_begin + blockIdx.x * 256 + threadIdx.x;
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
     const int k = KMAX == 1 ? 1 : p.k;
     const V* x = static_cast<const V*>(p.x);
     V* y = static_cast<V*>(p.y);
@@ -341,30 +406,27 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
     for (int j = 0; j < KMAX; ++j)
         dacc[j] = 0.0;
 
-    if (t < p.tile_end) {
-        const uint8_t tf = p.tile_flags[t];
-        if (!(tf & kStartsRow)) {
-            const bool closing =
-              (tf & kMultiSeg) || (t + 1 == p.tile_end) || (p.tile_flags[t + 1] & kStartsRow);
-            if (closing) {
-                int64_t kk = static_cast<int64_t>(t) - 1;
-                while (kk >= static_cast<int64_t>(p.tile_begin) && !(p.tile_flags[kk] & (kStartsRow | kMultiSeg)))
-                    --kk;
-                const uint32_t from = kk >= static_cast<int64_t>(p.tile_begin) ? static_cast<uint32_t>(kk) : p.tile_begin;
-                const uint32_t row = p.tile_ptr[t] & ~kDirty;
-                V* yr = y + static_cast<size_t>(row - p.row_base) * k;
+    if (c < p.nchunks) {
+        const uint8_t cf = p.chunk_flags[c];
+        if (cf & kChunkHead) {
+            // walk back over chunks that lie entirely inside this row
+            int64_t kk = static_cast<int64_t>(c) - 1;
+            while (kk >= 0 && !(p.chunk_flags[kk] & (kChunkStarts | kChunkHead)))
+                --kk;
+            const uint32_t from = kk >= 0 ? static_cast<uint32_t>(kk) : 0u;
+            const uint32_t row = p.tile_ptr[p.tile_begin + c * p.tpw] & ~kDirty;
+            V* yr = y + static_cast<size_t>(row - p.row_base) * k;
 #pragma unroll
-                for (int j = 0; j < KMAX; ++j) {
-                    if (j >= k)
-                        break;
+            for (int j = 0; j < KMAX; ++j) {
+                if (j < k) {
                     V s;
-                    if (from < t) {
+                    if (from < c) {
                         s = tail[static_cast<size_t>(from) * k + j];
-                        for (uint32_t u = from + 1; u < t; ++u)
+                        for (uint32_t u = from + 1; u < c; ++u)
                             s = s + tail[static_cast<size_t>(u) * k + j];
-                        s = s + head[static_cast<size_t>(t) * k + j];
+                        s = s + head[static_cast<size_t>(c) * k + j];
                     } else {
-                        s = head[static_cast<size_t>(t) * k + j];
+                        s = head[static_cast<size_t>(c) * k + j];
                     }
                     yr[j] = s;
                     if constexpr (DOT)
@@ -376,7 +438,7 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
 #pragma unroll
             for (int j = 0; j < KMAX; ++j)
                 if (j < k)
-                    dacc[j] += p.dot_tile[static_cast<size_t>(t) * k + j];
+                    dacc[j] += p.dot_chunk[static_cast<size_t>(c) * k + j];
         }
     }
 
@@ -404,45 +466,87 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
     }
 }
 
+// Launch geometry: one chunk per wavefront, as many wavefronts as the device
+// keeps resident for this kernel.
+struct Plan {
+    uint32_t tpw = 1, nchunks = 0, tgrid = 0, fgrid = 1;
+};
+
+int device_cus();
+
 template<typename V, int SIGMA, int KMAX, bool DOT>
-void launch_tiles(const TileParams& tp, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL((k_csr5_tiles<V, SIGMA, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp);
+Plan make_plan(uint32_t ntiles) {
+    static int blocks_per_cu = 0;
+    if (!blocks_per_cu) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_csr5_tiles<V, SIGMA, KMAX, DOT>, 256, 0) != hipSuccess ||
+            b <= 0)
+            b = 2;
+        blocks_per_cu = b;
+    }
+    Plan pl;
+    if (!ntiles)
+        return pl;
+    const uint64_t waves = static_cast<uint64_t>(device_cus()) * blocks_per_cu * 4;
+    pl.tpw = static_cast<uint32_t>((ntiles + waves - 1) / waves);
+    pl.nchunks = (ntiles + pl.tpw - 1) / pl.tpw;
+    pl.tgrid = (pl.nchunks + 3) / 4;
+    pl.fgrid = (pl.nchunks + 255) / 256;
+    return pl;
 }
 
 template<typename V, int KMAX, bool DOT>
-void launch_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, hipStream_t st) {
+Plan plan_by_sigma(uint32_t sigma, uint32_t ntiles) {
     switch (sigma) {
-        case 8: launch_tiles<V, 8, KMAX, DOT>(tp, grid, st); break;
-        case 16: launch_tiles<V, 16, KMAX, DOT>(tp, grid, st); break;
-        case 24: launch_tiles<V, 24, KMAX, DOT>(tp, grid, st); break;
-        default: launch_tiles<V, 32, KMAX, DOT>(tp, grid, st); break;
+        case 8: return make_plan<V, 8, KMAX, DOT>(ntiles);
+        case 16: return make_plan<V, 16, KMAX, DOT>(ntiles);
+        case 24: return make_plan<V, 24, KMAX, DOT>(ntiles);
+        default: return make_plan<V, 32, KMAX, DOT>(ntiles);
     }
 }
 
-template<typename V, int KMAX>
-void launch_pair(const DevFormat& A, const TileParams& tp, const FixParams& fp, uint32_t tgrid, uint32_t fgrid,
-                 bool dot, hipStream_t st) {
-    if (tgrid) {
-        if (dot)
-            launch_by_sigma<V, KMAX, true>(A.sigma, tp, tgrid, st);
-        else
-            launch_by_sigma<V, KMAX, false>(A.sigma, tp, tgrid, st);
+template<typename V, int KMAX, bool DOT>
+void launch_tiles_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, hipStream_t st) {
+    switch (sigma) {
+        case 8: hipLaunchKernelGGL((k_csr5_tiles<V, 8, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
+        case 16: hipLaunchKernelGGL((k_csr5_tiles<V, 16, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
+        case 24: hipLaunchKernelGGL((k_csr5_tiles<V, 24, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
+        default: hipLaunchKernelGGL((k_csr5_tiles<V, 32, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
     }
+}
+
+// Launches the tile kernel and the fix-up kernel; records `mid` between them
+// when given. Returns the number of fix-up blocks (dot partials).
+template<typename V, int KMAX>
+uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, hipStream_t st, hipEvent_t mid) {
+    const uint32_t ntiles = tp.tile_end - tp.tile_begin;
+    const Plan pl = dot ? plan_by_sigma<V, KMAX, true>(A.sigma, ntiles) : plan_by_sigma<V, KMAX, false>(A.sigma, ntiles);
+    tp.tpw = fp.tpw = pl.tpw;
+    tp.nchunks = fp.nchunks = pl.nchunks;
+    if (pl.tgrid) {
+        if (dot)
+            launch_tiles_by_sigma<V, KMAX, true>(A.sigma, tp, pl.tgrid, st);
+        else
+            launch_tiles_by_sigma<V, KMAX, false>(A.sigma, tp, pl.tgrid, st);
+    }
+    if (mid)
+        (void)hipEventRecord(mid, st);
     if (dot)
-        hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, true>), dim3(fgrid), dim3(256), 0, st, fp);
+        hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, true>), dim3(pl.fgrid), dim3(256), 0, st, fp);
     else
-        hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, false>), dim3(fgrid), dim3(256), 0, st, fp);
+        hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, false>), dim3(pl.fgrid), dim3(256), 0, st, fp);
+    return pl.fgrid;
 }
 
 // Multi-RHS launchers, one translation unit per k bound (spmv_k*.hip).
-void launch_pair_k2(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, uint32_t tgrid,
-                    uint32_t fgrid, bool dot, hipStream_t st);
-void launch_pair_k4(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, uint32_t tgrid,
-                    uint32_t fgrid, bool dot, hipStream_t st);
-void launch_pair_k8(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, uint32_t tgrid,
-                    uint32_t fgrid, bool dot, hipStream_t st);
-void launch_pair_k16(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, uint32_t tgrid,
-                     uint32_t fgrid, bool dot, hipStream_t st);
+uint32_t launch_pair_k2(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, bool dot,
+                        hipStream_t st);
+uint32_t launch_pair_k4(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, bool dot,
+                        hipStream_t st);
+uint32_t launch_pair_k8(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, bool dot,
+                        hipStream_t st);
+uint32_t launch_pair_k16(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, bool dot,
+                         hipStream_t st);
 
 } // namespace detail
 } // namespace dimgpu

@@ -4,12 +4,9 @@
 namespace dimgpu {
 namespace detail {
 
-void launch_pair_k16(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, uint32_t tgrid,
-                    uint32_t fgrid, bool dot, hipStream_t st) {
-    if (f32)
-        launch_pair<float, 16>(A, tp, fp, tgrid, fgrid, dot, st);
-    else
-        launch_pair<double, 16>(A, tp, fp, tgrid, fgrid, dot, st);
+uint32_t launch_pair_k16(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, bool dot,
+                        hipStream_t st) {
+    return f32 ? launch_pair<float, 16>(A, tp, fp, dot, st, nullptr) : launch_pair<double, 16>(A, tp, fp, dot, st, nullptr);
 }
 
 } // namespace detail

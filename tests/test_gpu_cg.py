@@ -1,0 +1,152 @@
+"""GPU conjugate gradient against the oracle's textbook CG (same stop rule):
+iteration counts within +-2, residual histories, reproducibility, error
+paths, multi-RHS and the FP32-refined mode."""
+import numpy as np
+import pytest
+
+from gpu_util import to_dev
+from diploma_research_amd import Csr5, DimError, conjugate_gradient, conjugate_gradient_multi, matgen
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(rp, ci, v, n, seed):
+    xs = np.random.default_rng(seed).uniform(-1, 1, n)
+    b = O.csr_spmv(rp, ci, v, xs)
+    return xs, b
+
+
+@pytest.mark.parametrize("sigma", [8, 16, 32])
+def test_cg_poisson2d_256_matches_oracle(cuda, sigma):
+    """2-D 5-point Laplacian 256^2 (the CLI workload), rtol 1e-10."""
+    rp, ci, v = matgen.poisson2d(256)
+    n = 65536
+    xs, b = _problem(rp, ci, v, n, 3)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=sigma)
+    x, res = conjugate_gradient(A, to_dev(b), rtol=1e-10, max_iters=5000, history=True)
+    oc = O.Csr5(n, n, rp, ci, v).cg(b, rtol=1e-10, maxit=5000, history=True)
+    assert res.converged and oc["converged"]
+    assert abs(res.iterations - oc["iterations"]) <= 2
+    assert res.relres <= 1e-10 and res.relres_true <= 1e-9
+    m = min(len(res.history), len(oc["history"]))
+    # the residual histories agree while rounding differences stay small
+    np.testing.assert_allclose(res.history[:40], oc["history"][:40], rtol=1e-8)
+    assert np.abs(np.log10(res.history[:m]) - np.log10(oc["history"][:m])).max() < 0.5
+    np.testing.assert_allclose(x.cpu().numpy(), xs, atol=1e-7)
+
+
+def test_cg_poisson3d_128_full_size(cuda):
+    """The benchmark operator: 128^3 7-point Poisson, b = A x*, rtol 1e-8,
+    max 5000 iterations: same iteration count as the oracle within +-2."""
+    rp, ci, v = matgen.poisson3d(128)
+    n = 128 ** 3
+    xs, b = _problem(rp, ci, v, n, 42)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=16)
+    x, res = conjugate_gradient(A, to_dev(b), rtol=1e-8, max_iters=5000)
+    oc = O.Csr5(n, n, rp, ci, v).cg(b, rtol=1e-8, maxit=5000, nthreads=16)
+    assert res.converged and oc["converged"]
+    assert abs(res.iterations - oc["iterations"]) <= 2, (res.iterations, oc["iterations"])
+    assert res.relres_true <= 2e-8
+    err = np.linalg.norm(x.cpu().numpy() - oc["x"]) / np.linalg.norm(oc["x"])
+    assert err < 1e-6
+
+
+def test_cg_is_bitwise_reproducible(cuda):
+    rp, ci, v = matgen.poisson3d(40)
+    n = 40 ** 3
+    _, b = _problem(rp, ci, v, n, 1)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=8)
+    bd = to_dev(b)
+    x1, r1 = conjugate_gradient(A, bd, rtol=1e-10)
+    x2, r2 = conjugate_gradient(A, bd, rtol=1e-10)
+    assert r1.iterations == r2.iterations
+    assert bool((x1 == x2).all())
+
+
+def test_cg_diagonal_distinct_eigenvalues(cuda):
+    n = 4096
+    d = np.repeat([1.0, 3.0, 7.0], [1365, 1365, 1366])
+    A = Csr5.from_csr((n, n, np.arange(n + 1, dtype=np.uint32), np.arange(n, dtype=np.uint32), d))
+    b = np.random.default_rng(0).uniform(-1, 1, n)
+    x, res = conjugate_gradient(A, to_dev(b), rtol=1e-12, max_iters=50)
+    assert res.converged and res.iterations == 3
+    np.testing.assert_allclose(x.cpu().numpy(), b / d, rtol=1e-12)
+
+
+def test_cg_initial_guess_and_zero_rhs(cuda):
+    import torch
+
+    rp, ci, v = matgen.poisson2d(64)
+    n = 4096
+    xs, b = _problem(rp, ci, v, n, 5)
+    A = Csr5.from_csr((n, n, rp, ci, v))
+    x0 = to_dev(xs)  # exact solution as x0: converged before the first iteration
+    x, res = conjugate_gradient(A, to_dev(b), x=x0.clone(), rtol=1e-8)
+    assert res.converged and res.iterations == 0
+    x, res = conjugate_gradient(A, torch.zeros(n, dtype=torch.float64, device="cuda"), rtol=1e-8)
+    assert res.converged and res.iterations == 0 and float(x.abs().max()) == 0.0
+    # a nonzero x0 continues from there, like the oracle
+    x0 = np.random.default_rng(9).uniform(-1, 1, n)
+    x, res = conjugate_gradient(A, to_dev(b), x=to_dev(x0), rtol=1e-10)
+    oc = O.Csr5(n, n, rp, ci, v).cg(b, x0=x0, rtol=1e-10)
+    assert res.converged and abs(res.iterations - oc["iterations"]) <= 2
+
+
+def test_cg_breakdown_on_indefinite_matrix(cuda):
+    n = 1000
+    d = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    A = Csr5.from_csr((n, n, np.arange(n + 1, dtype=np.uint32), np.arange(n, dtype=np.uint32), d))
+    b = np.ones(n)
+    with pytest.raises(DimError, match="breakdown"):
+        conjugate_gradient(A, to_dev(b), rtol=1e-10)
+
+
+def test_cg_iteration_cap(cuda):
+    rp, ci, v = matgen.poisson2d(128)
+    n = 128 * 128
+    _, b = _problem(rp, ci, v, n, 2)
+    A = Csr5.from_csr((n, n, rp, ci, v))
+    x, res = conjugate_gradient(A, to_dev(b), rtol=1e-14, max_iters=37)
+    assert not res.converged and res.iterations == 37
+    oc = O.Csr5(n, n, rp, ci, v).cg(b, rtol=1e-14, maxit=37)
+    np.testing.assert_allclose(x.cpu().numpy(), oc["x"], rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("k", [2, 3, 8])
+def test_cg_multi_rhs_equals_single(cuda, k):
+    """k right-hand sides through one SpMM per iteration give, column by
+    column, the same iterates as k single solves (bitwise)."""
+    import torch
+
+    rp, ci, v = matgen.varcoef((24, 24, 24), 1.0, seed=3)
+    n = 24 ** 3
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=16)
+    B = np.random.default_rng(k).uniform(-1, 1, (n, k))
+    X, res = conjugate_gradient_multi(A, to_dev(B), rtol=1e-10, max_iters=3000)
+    for j in range(k):
+        xj, rj = conjugate_gradient(A, to_dev(B[:, j].copy()), rtol=1e-10, max_iters=3000)
+        assert res[j].converged and res[j].iterations == rj.iterations
+        assert torch.equal(X[:, j].contiguous(), xj)
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_cg_fp32_refined_reaches_fp64_accuracy(cuda, k):
+    """FP32 iteration with FP64 residual replacement reaches relative
+    residual 1e-10 on a variable-coefficient operator (lognormal(0,1)
+    diffusivity); the error against the FP64 oracle solution is stated."""
+    rp, ci, v = matgen.varcoef((32, 32, 32), 1.0, seed=11)
+    n = 32 ** 3
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=16, keep_f32=True)
+    B = np.random.default_rng(7).uniform(-1, 1, (n, k))
+    X, res = conjugate_gradient_multi(A, to_dev(B), rtol=1e-10, max_iters=20000, precision="f32-refined")
+    O64 = O.Csr5(n, n, rp, ci, v)
+    for j in range(k):
+        oc = O64.cg(B[:, j].copy(), rtol=1e-10, maxit=20000)
+        assert res[j].converged, res[j]
+        assert res[j].relres_true <= 1.01e-10
+        assert res[j].refinements >= 1
+        # FP32 iterations cost some extra steps but stay close to FP64 CG
+        assert res[j].iterations <= 1.5 * oc["iterations"] + 20, (res[j].iterations, oc["iterations"])
+        err = np.linalg.norm(X[:, j].cpu().numpy() - oc["x"]) / np.linalg.norm(oc["x"])
+        assert err < 1e-6, err
