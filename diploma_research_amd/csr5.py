@@ -146,7 +146,7 @@ class Csr5:
     def from_csr(cls, csr, sigma: int = 16, keep_f32: bool = False) -> "Csr5":
         """csr5::from_csr (csr5.h:791-803). `csr` is a Csr or a (rows, cols,
         row_ptr, col_idx, vals) tuple of host arrays. sigma=0 runs the
-        autotuner (times sigma in {8, 16, 24, 32} on this matrix)."""
+        autotuner (times the candidate sigmas on this matrix)."""
         if isinstance(csr, Csr):
             rows, cols = csr.dimensions.rows, csr.dimensions.cols
             rp, ci, v = csr.row_start_offsets, csr.col_indices, csr.values

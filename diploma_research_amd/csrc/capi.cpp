@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -106,9 +107,34 @@ dim_status check_handle(const dim_csr5* A) {
 // Times candidate tile heights on this matrix and keeps the fastest:
 // sigma trades per-tile fixed work (descriptor decode, wavefront scan, carry
 // slots) against registers per lane and load-balance granularity.
+// Candidate tile heights: the supported sigma nearest to 1..4 times the mean
+// row length (a lane then tends to start on a row and keep one row per step
+// of the wavefront, so the x gathers of a step are close together), plus 4,
+// 8 and 16 for irregular rows.
+std::vector<uint32_t> sigma_candidates(uint32_t rows, uint64_t nnz) {
+    static const uint32_t supported[] = {
+#define DIM_SIGMA_ITEM(S) S,
+      DIM_FOR_EACH_SIGMA(DIM_SIGMA_ITEM)
+#undef DIM_SIGMA_ITEM
+    };
+    std::vector<uint32_t> c = {4, 8, 16};
+    const double len = rows ? static_cast<double>(nnz) / rows : 8.0;
+    for (int m = 1; m <= 4; ++m) {
+        const double want = m * len;
+        uint32_t best = supported[0];
+        for (uint32_t s : supported)
+            if (std::abs(static_cast<double>(s) - want) < std::abs(static_cast<double>(best) - want))
+                best = s;
+        c.push_back(best);
+    }
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+    return c;
+}
+
 dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
                           const uint32_t* col_idx, const double* vals, bool keep_f32, dim_csr5** out) {
-    static const uint32_t candidates[] = {8, 16, 24, 32};
+    const std::vector<uint32_t> candidates = sigma_candidates(rows, nnz);
     dim_csr5* best = nullptr;
     float best_ms = 0;
     double* x = nullptr;

@@ -19,9 +19,18 @@ constexpr uint32_t kDirty = 0x80000000u;
 constexpr uint8_t kStartsRow = 1; // the tile's first element is the first element of a row
 constexpr uint8_t kMultiSeg = 2;  // the tile holds more than one row segment
 
-inline bool sigma_supported(uint32_t s) { return s == 8 || s == 16 || s == 24 || s == 32; }
+// Tile heights with compiled kernels. Small and odd values let the autotuner
+// match sigma to short, regular rows (a 7-point stencil at sigma = 7 keeps a
+// lane on one row, so the x gathers of a wavefront step are contiguous).
+#define DIM_FOR_EACH_SIGMA(X) X(4) X(5) X(6) X(7) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(32)
+inline bool sigma_supported(uint32_t s) {
+#define DIM_SIGMA_CASE(S) if (s == S) return true;
+    DIM_FOR_EACH_SIGMA(DIM_SIGMA_CASE)
+#undef DIM_SIGMA_CASE
+    return false;
+}
 
-// Host image of the GPU CSR5 format (omega = 64, sigma in {8,16,24,32}).
+// Host image of the GPU CSR5 format (omega = 64, sigma from DIM_FOR_EACH_SIGMA).
 //
 //  * tiles = ceil(nnz / (64*sigma)); the last tile is padded (val 0, col 0)
 //    instead of keeping the reference's separate CSR tail partition.

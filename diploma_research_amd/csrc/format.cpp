@@ -68,7 +68,7 @@ std::string validate_csr(uint32_t rows, uint32_t cols, uint64_t nnz, const uint3
 std::string build_format(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
                          const uint32_t* col_idx, const double* vals, uint32_t sigma, HostFormat& f) {
     if (!sigma_supported(sigma))
-        return "sigma must be one of 8, 16, 24, 32";
+        return "sigma must be one of 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32";
     f = HostFormat{};
     f.rows = rows;
     f.cols = cols;

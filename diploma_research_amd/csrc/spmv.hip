@@ -231,23 +231,17 @@ hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, vo
                            static_cast<size_t>(A.cols), 0.0);
     if (A.tiles) {
         const uint32_t grid = (A.tiles + 3) / 4;
-#define DIM_T(V, S) hipLaunchKernelGGL((k_csr5_transpose<V, S>), dim3(grid), dim3(256), 0, stream, tp)
+#define DIM_T_F(S)                                                                                                    \
+    case S: hipLaunchKernelGGL((k_csr5_transpose<float, S>), dim3(grid), dim3(256), 0, stream, tp); break;
+#define DIM_T_D(S)                                                                                                    \
+    case S: hipLaunchKernelGGL((k_csr5_transpose<double, S>), dim3(grid), dim3(256), 0, stream, tp); break;
         if (f32) {
-            switch (A.sigma) {
-                case 8: DIM_T(float, 8); break;
-                case 16: DIM_T(float, 16); break;
-                case 24: DIM_T(float, 24); break;
-                default: DIM_T(float, 32); break;
-            }
+            switch (A.sigma) { DIM_FOR_EACH_SIGMA(DIM_T_F) default: break; }
         } else {
-            switch (A.sigma) {
-                case 8: DIM_T(double, 8); break;
-                case 16: DIM_T(double, 16); break;
-                case 24: DIM_T(double, 24); break;
-                default: DIM_T(double, 32); break;
-            }
+            switch (A.sigma) { DIM_FOR_EACH_SIGMA(DIM_T_D) default: break; }
         }
-#undef DIM_T
+#undef DIM_T_F
+#undef DIM_T_D
     }
     return hipGetLastError();
 }
@@ -259,10 +253,11 @@ hipError_t launch_sampled_outer(const DevFormat& A, double scale, const double* 
     TileParams tp = make_tile_params(A, false);
     const uint32_t grid = (A.tiles + 3) / 4;
     switch (A.sigma) {
-        case 8: hipLaunchKernelGGL((k_sampled_outer<8>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
-        case 16: hipLaunchKernelGGL((k_sampled_outer<16>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
-        case 24: hipLaunchKernelGGL((k_sampled_outer<24>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
-        default: hipLaunchKernelGGL((k_sampled_outer<32>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
+#define DIM_O(S)                                                                                                      \
+    case S: hipLaunchKernelGGL((k_sampled_outer<S>), dim3(grid), dim3(256), 0, stream, tp, scale, u, v, out); break;
+        DIM_FOR_EACH_SIGMA(DIM_O)
+#undef DIM_O
+        default: break;
     }
     return hipGetLastError();
 }

@@ -29,20 +29,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace dimgpu {
 namespace detail {
 
+// bit flags: one bit per step of a lane's column
 template<int SIGMA>
 struct DescWord {
-    using type = uint32_t;
-};
-template<>
-struct DescWord<8> {
-    using type = uint16_t;
-};
-template<>
-struct DescWord<16> {
-    using type = uint16_t;
+    using type = typename std::conditional<(SIGMA <= 16), uint16_t, uint32_t>::type;
 };
 
 // Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD works on
@@ -53,14 +48,86 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
     return x * q + (x < r ? x : r) + i;
 }
 
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int u = __shfl_up(v, d, 64);
-        if (lane >= d)
-            v += u;
-    }
+// ---- wavefront scans on DPP (data-parallel primitives: lane moves done by
+// the VALU itself, no LDS round trip). Inclusive scans over the 64 lanes use
+// row_shr 1,2,4,8 inside each row of 16 lanes, then row_bcast:15 and
+// row_bcast:31 to carry row totals into the rows above.
+constexpr int kDppRowShr1 = 0x111;
+constexpr int kDppRowShr2 = 0x112;
+constexpr int kDppRowShr4 = 0x114;
+constexpr int kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+template<int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_int(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, true);
+}
+template<int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_val(float v) {
+    return __int_as_float(dpp_int<CTRL, ROW_MASK>(__float_as_int(v)));
+}
+template<int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_val(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_int<CTRL, ROW_MASK>(static_cast<int>(b & 0xffffffffll));
+    const int hi = dpp_int<CTRL, ROW_MASK>(static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+template<int CTRL, int ROW_MASK>
+__device__ __forceinline__ void scan_step(int& v) {
+    v += dpp_int<CTRL, ROW_MASK>(v);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v, int) {
+    scan_step<kDppRowShr1, 0xf>(v);
+    scan_step<kDppRowShr2, 0xf>(v);
+    scan_step<kDppRowShr4, 0xf>(v);
+    scan_step<kDppRowShr8, 0xf>(v);
+    scan_step<kDppRowBcast15, 0xa>(v);
+    scan_step<kDppRowBcast31, 0xc>(v);
     return v;
+}
+
+// One step of a segmented inclusive scan: (a, f) = (up_a, up_f) (+) (a, f),
+// where f marks a lane that begins a segment. Lanes without a source see
+// (0, 0), which leaves them unchanged.
+template<int CTRL, int ROW_MASK, typename V, int KMAX>
+__device__ __forceinline__ void seg_scan_step(V (&a)[KMAX], int& f, int k) {
+    const int uf = dpp_int<CTRL, ROW_MASK>(f);
+    V up[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+        up[j] = (j < k) ? dpp_val<CTRL, ROW_MASK>(a[j]) : V(0);
+    if (!f) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j)
+            a[j] = up[j] + a[j];
+        f = uf;
+    }
+}
+
+template<typename V, int KMAX>
+__device__ __forceinline__ void wave_seg_scan(V (&a)[KMAX], int& f, int k) {
+    seg_scan_step<kDppRowShr1, 0xf>(a, f, k);
+    seg_scan_step<kDppRowShr2, 0xf>(a, f, k);
+    seg_scan_step<kDppRowShr4, 0xf>(a, f, k);
+    seg_scan_step<kDppRowShr8, 0xf>(a, f, k);
+    seg_scan_step<kDppRowBcast15, 0xa>(a, f, k);
+    seg_scan_step<kDppRowBcast31, 0xc>(a, f, k);
+}
+
+// value of a wave-uniform lane, as a scalar
+__device__ __forceinline__ int lane_value(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float lane_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -70,9 +137,12 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Matrix values and column indices are read once per product. Plain loads
+// measured faster than non-temporal ones here (the gathers of x share the
+// same path through L2 and the Infinity Cache).
 template<typename V>
 __device__ __forceinline__ V ldg_stream(const V* p) {
-    return __builtin_nontemporal_load(p);
+    return *p;
 }
 
 // per-chunk flags written by the tile kernel
@@ -106,7 +176,9 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     if (p.done && *p.done)
         return;
     const int lane = threadIdx.x & 63;
-    const uint32_t c = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    // the chunk index is uniform across the wavefront: keep it (and every
+    // per-tile quantity derived from it) in scalar registers
+    const uint32_t c = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
     if (c >= p.nchunks)
         return; // wave-uniform
     const uint32_t t0 = p.tile_begin + c * p.tpw;
@@ -173,7 +245,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         const int nb = __popc(bits);
         const int incl = wave_incl_scan(nb, lane);
         const int y_off = incl - nb; // first segment that starts in this lane
-        const int nseg = __shfl(incl, 63, 64);
+        const int nseg = lane_value(incl, 63);
         const uint64_t started = __ballot(nb > 0);
 
         auto write_row = [&](uint32_t row, const V* val) {
@@ -275,20 +347,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
 #pragma unroll
         for (int j = 0; j < KMAX; ++j)
             A[j] = red[j];
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int fup = __shfl_up(f, d, 64);
-            V up[KMAX];
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                up[j] = __shfl_up(A[j], d, 64);
-            if (lane >= d && !f) {
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    A[j] = up[j] + A[j];
-                f = fup;
-            }
-        }
+        wave_seg_scan<V, KMAX>(A, f, k);
 
         // A lane in which a row starts closes the row left open by the
         // previous such lane: its trailing (blue) partial plus the scanned reds.
@@ -311,10 +370,13 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         V tv[KMAX];
 #pragma unroll
         for (int j = 0; j < KMAX; ++j) {
-            const V bl = __shfl(blue[j], L, 64);
-            const V a63 = __shfl(A[j], 63, 64);
-            const V b63 = __shfl(blue[j], 63, 64);
-            tv[j] = (L == 63) ? b63 : bl + a63;
+            if (j < k) {
+                const V bl = lane_value(blue[j], L);
+                const V a63 = lane_value(A[j], 63);
+                tv[j] = (L == 63) ? bl : bl + a63;
+            } else {
+                tv[j] = V(0);
+            }
         }
         if (nseg == 1 && !starts_row) {
             // the whole tile continues the carried row
@@ -498,20 +560,22 @@ Plan make_plan(uint32_t ntiles) {
 template<typename V, int KMAX, bool DOT>
 Plan plan_by_sigma(uint32_t sigma, uint32_t ntiles) {
     switch (sigma) {
-        case 8: return make_plan<V, 8, KMAX, DOT>(ntiles);
-        case 16: return make_plan<V, 16, KMAX, DOT>(ntiles);
-        case 24: return make_plan<V, 24, KMAX, DOT>(ntiles);
-        default: return make_plan<V, 32, KMAX, DOT>(ntiles);
+#define DIM_PLAN_CASE(S)                                                                                              \
+    case S: return make_plan<V, S, KMAX, DOT>(ntiles);
+        DIM_FOR_EACH_SIGMA(DIM_PLAN_CASE)
+#undef DIM_PLAN_CASE
+        default: return Plan{};
     }
 }
 
 template<typename V, int KMAX, bool DOT>
 void launch_tiles_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, hipStream_t st) {
     switch (sigma) {
-        case 8: hipLaunchKernelGGL((k_csr5_tiles<V, 8, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
-        case 16: hipLaunchKernelGGL((k_csr5_tiles<V, 16, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
-        case 24: hipLaunchKernelGGL((k_csr5_tiles<V, 24, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
-        default: hipLaunchKernelGGL((k_csr5_tiles<V, 32, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
+#define DIM_TILES_CASE(S)                                                                                             \
+    case S: hipLaunchKernelGGL((k_csr5_tiles<V, S, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
+        DIM_FOR_EACH_SIGMA(DIM_TILES_CASE)
+#undef DIM_TILES_CASE
+        default: break;
     }
 }
 

@@ -9,7 +9,7 @@ from diploma_research_amd import Csr5, matgen
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
-SIGMAS = [8, 16, 24, 32]
+SIGMAS = [4, 7, 8, 16, 24, 32]
 
 
 @pytest.mark.parametrize("name", ["mm", "rand_dirty", "rand_longrows", "rect_exact"])
