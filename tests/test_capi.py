@@ -49,7 +49,7 @@ def test_invalid_csr_rejected_before_touching_the_gpu():
     ci = np.array([0, 5], np.uint32)  # out of range column
     st = L.dim_csr5_from_csr(2, 2, 2, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, None, C.byref(h))
     assert st == N.DIM_INVALID_ARGUMENT
-    opts = N.Csr5Opts(12, 0)  # unsupported sigma
+    opts = N.Csr5Opts(9, 0)  # unsupported sigma
     ci = np.array([0, 1], np.uint32)
     st = L.dim_csr5_from_csr(2, 2, 2, rp.ctypes.data, ci.ctypes.data, v.ctypes.data, C.byref(opts), C.byref(h))
     assert st == N.DIM_INVALID_ARGUMENT
