@@ -164,6 +164,43 @@ class Csr5:
         return cls(h.value)
 
     @classmethod
+    def from_csr_batched(cls, systems, sigma: int = 0, keep_f32: bool = False) -> "Csr5":
+        """A block-diagonal batch of square systems, each (rows, row_ptr,
+        col_idx, vals) on the host; sigma=0 autotunes on the whole batch."""
+        n = len(systems)
+        keep = []
+        rows = (C.c_uint32 * n)()
+        nnz = (C.c_uint64 * n)()
+        rps = (C.c_void_p * n)()
+        cis = (C.c_void_p * n)()
+        vs = (C.c_void_p * n)()
+        for i, (r, rp, ci, v) in enumerate(systems):
+            rp = np.ascontiguousarray(rp, dtype=np.uint32)
+            ci = np.ascontiguousarray(ci, dtype=np.uint32)
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            keep += [rp, ci, v]
+            rows[i], nnz[i] = int(r), len(v)
+            rps[i], cis[i], vs[i] = rp.ctypes.data, ci.ctypes.data, v.ctypes.data
+        opts = N.Csr5Opts(sigma, N.DIM_CSR5_KEEP_F32 if keep_f32 else 0)
+        h = C.c_void_p()
+        N.check(N.lib().dim_csr5_from_csr_batched(n, rows, nnz, rps, cis, vs, C.byref(opts), C.byref(h)))
+        A = cls(h.value)
+        A.system_rows = np.array([rows[i] for i in range(n)], dtype=np.int64)
+        A.system_nnz = np.array([nnz[i] for i in range(n)], dtype=np.int64)
+        return A
+
+    @property
+    def n_systems(self) -> int:
+        return int(N.lib().dim_csr5_n_systems(self._h))
+
+    def set_values(self, vals, stream=None):
+        """Replaces the stored values (same pattern) from a float64 GPU tensor
+        in the CSR order the matrix was built from."""
+        if vals.numel() != self.non_zero_count() or not vals.is_contiguous():
+            raise ValueError("vals must be a contiguous tensor with one entry per stored element")
+        N.check(N.lib().dim_csr5_set_values(self._h, _ptr(vals), _stream_ptr(stream)))
+
+    @classmethod
     def from_csr_device(cls, rows, cols, row_ptr, col_idx, vals, sigma=16, keep_f32=False, stream=None):
         """Same, with the CSR arrays already on the GPU (torch tensors:
         int32/uint32 row_ptr and col_idx, float64 vals); conversion runs on
@@ -308,11 +345,11 @@ class CgResult:
     history: Optional[np.ndarray] = field(default=None, repr=False)
 
 
-def _cg_opts(rtol, max_iters, precision, check_every, refine_delta, hist):
+def _cg_opts(rtol, max_iters, precision, check_every, refine_delta, hist, flags=0):
     prec = {"f64": N.DIM_CG_F64, "fp64": N.DIM_CG_F64, "f32-refined": N.DIM_CG_F32_REFINED,
             "fp32-ir": N.DIM_CG_F32_REFINED}[precision]
     return N.CgOpts(rtol, max_iters, prec, check_every, refine_delta,
-                    hist.ctypes.data if hist is not None else None)
+                    hist.ctypes.data if hist is not None else None, flags)
 
 
 def _result(r, hist=None):
@@ -349,5 +386,22 @@ def conjugate_gradient_multi(A: Csr5, B, X=None, rtol=1e-8, max_iters=5000, prec
     opts = _cg_opts(rtol, max_iters, precision, check_every, refine_delta, None)
     res = (N.CgResult * k)()
     N.check(N.lib().dim_cg_multi(A._h, k, _ptr(B), _ptr(X), C.byref(opts), res, _stream_ptr(stream)),
+            allow=(N.DIM_NOT_CONVERGED,))
+    return X, [_result(r) for r in res]
+
+
+def conjugate_gradient_batched(A: Csr5, B, X=None, rtol=1e-8, max_iters=5000, check_every=0, transpose=False,
+                               stream=None):
+    """Solves every system of a batch (A from Csr5.from_csr_batched; B, X:
+    float64 GPU tensors holding the systems' vectors back to back). With
+    transpose=True the products use A^T (the adjoint solve). Returns (X,
+    [CgResult per system])."""
+    torch = _torch()
+    if X is None:
+        X = torch.zeros_like(B)
+    opts = _cg_opts(rtol, max_iters, "f64", check_every, 0.0, None, N.DIM_CG_TRANSPOSE if transpose else 0)
+    S = A.n_systems
+    res = (N.CgResult * S)()
+    N.check(N.lib().dim_cg_batched(A._h, _ptr(B), _ptr(X), C.byref(opts), res, _stream_ptr(stream)),
             allow=(N.DIM_NOT_CONVERGED,))
     return X, [_result(r) for r in res]

@@ -91,22 +91,12 @@ dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out) {
     DIM_HIP_TRY(dev_alloc(h.get(), s.dot_blk, ((T + 255) / 256) * kMaxK));
     h->sys_row = {0, f.rows};
     h->sys_tile = {0, f.tiles};
-    h->sys_nnz_csr = {0, static_cast<uint32_t>(f.nnz)};
+    h->sys_nnz_csr = {0, f.nnz};
+    h->nnz_logical = f.nnz;
     *out = h.release();
     return DIM_SUCCESS;
 }
 
-namespace {
-
-dim_status check_handle(const dim_csr5* A) {
-    if (!A)
-        return fail(DIM_INVALID_ARGUMENT, "matrix handle is null");
-    return DIM_SUCCESS;
-}
-
-// Times candidate tile heights on this matrix and keeps the fastest:
-// sigma trades per-tile fixed work (descriptor decode, wavefront scan, carry
-// slots) against registers per lane and load-balance granularity.
 // Candidate tile heights: the supported sigma nearest to 1..4 times the mean
 // row length (a lane then tends to start on a row and keep one row per step
 // of the wavefront, so the x gathers of a step are close together), plus 4,
@@ -132,6 +122,34 @@ std::vector<uint32_t> sigma_candidates(uint32_t rows, uint64_t nnz) {
     return c;
 }
 
+DevFormat system_view(const dim_csr5* A, uint32_t s) {
+    DevFormat view = A->f;
+    const uint32_t t0 = A->sys_tile[s], t1 = A->sys_tile[s + 1];
+    const size_t tile = size_t{kOmega} * A->f.sigma;
+    view.tiles = t1 - t0;
+    view.tile_ptr = A->f.tile_ptr + t0;
+    view.bit_flag = static_cast<char*>(A->f.bit_flag) + size_t{t0} * kOmega * (A->f.sigma <= 16 ? 2 : 4);
+    view.tile_flags = A->f.tile_flags + t0;
+    view.tile_desc_offset_ptr = A->f.tile_desc_offset_ptr + t0;
+    view.col_idx = A->f.col_idx + t0 * tile;
+    view.vals64 = A->f.vals64 + t0 * tile;
+    view.vals32 = A->f.vals32 ? A->f.vals32 + t0 * tile : nullptr;
+    view.nnz = A->sys_nnz_csr[s + 1] - A->sys_nnz_csr[s];
+    view.padded = size_t{view.tiles} * tile;
+    return view;
+}
+
+namespace {
+
+dim_status check_handle(const dim_csr5* A) {
+    if (!A)
+        return fail(DIM_INVALID_ARGUMENT, "matrix handle is null");
+    return DIM_SUCCESS;
+}
+
+// Times candidate tile heights on this matrix and keeps the fastest:
+// sigma trades per-tile fixed work (descriptor decode, wavefront scan, carry
+// slots) against registers per lane and load-balance granularity.
 dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
                           const uint32_t* col_idx, const double* vals, bool keep_f32, dim_csr5** out) {
     const std::vector<uint32_t> candidates = sigma_candidates(rows, nnz);
@@ -261,7 +279,7 @@ dim_status dim_csr5_get_info(const dim_csr5* A, dim_csr5_info* info) {
         return fail(DIM_INVALID_ARGUMENT, "info is null");
     info->rows = A->f.rows;
     info->cols = A->f.cols;
-    info->nnz = A->f.nnz;
+    info->nnz = A->nnz_logical;
     info->sigma = A->f.sigma;
     info->omega = kOmega;
     info->tile_count = A->f.tiles;
@@ -444,22 +462,24 @@ dim_status dim_csr5_sampled_outer(const dim_csr5* A, double scale, const double*
     if ((A->f.rows && !u) || (A->f.cols && !v) || (A->f.nnz && !out))
         return fail(DIM_INVALID_ARGUMENT, "null vector");
     for (uint32_t s = 0; s < A->n_systems; ++s) {
-        DevFormat view = A->f;
-        const uint32_t t0 = A->sys_tile[s], t1 = A->sys_tile[s + 1];
-        const size_t tile = size_t{kOmega} * A->f.sigma;
-        view.tiles = t1 - t0;
-        view.tile_ptr = A->f.tile_ptr + t0;
-        view.bit_flag = static_cast<char*>(A->f.bit_flag) + size_t{t0} * kOmega * (A->f.sigma <= 16 ? 2 : 4);
-        view.tile_flags = A->f.tile_flags + t0;
-        view.tile_desc_offset_ptr = A->f.tile_desc_offset_ptr + t0;
-        view.col_idx = A->f.col_idx + t0 * tile;
-        view.vals64 = A->f.vals64 + t0 * tile;
-        view.nnz = A->sys_nnz_csr[s + 1] - A->sys_nnz_csr[s];
+        const DevFormat view = system_view(A, s);
         DIM_HIP_TRY(launch_sampled_outer(view, scale, u, v, out + A->sys_nnz_csr[s], static_cast<hipStream_t>(stream)));
     }
     return DIM_SUCCESS;
 }
 
 uint32_t dim_csr5_n_systems(const dim_csr5* A) { return A ? A->n_systems : 0; }
+
+dim_status dim_csr5_set_values(dim_csr5* A, const double* vals, void* stream) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (A->nnz_logical && !vals)
+        return fail(DIM_INVALID_ARGUMENT, "vals is null");
+    for (uint32_t s = 0; s < A->n_systems; ++s) {
+        const DevFormat view = system_view(A, s);
+        DIM_HIP_TRY(launch_set_values(view, vals + A->sys_nnz_csr[s], static_cast<hipStream_t>(stream)));
+    }
+    return DIM_SUCCESS;
+}
 
 } // extern "C"

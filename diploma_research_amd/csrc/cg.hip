@@ -658,15 +658,4 @@ dim_status dim_cg_multi(const dim_csr5* A, uint32_t k, const double* B, double* 
     return run(const_cast<dim_csr5*>(A), static_cast<int>(k), B, X, opts, res, static_cast<hipStream_t>(stream));
 }
 
-dim_status dim_csr5_from_csr_batched(uint32_t, const uint32_t*, const uint64_t*, const uint32_t* const*,
-                                     const uint32_t* const*, const double* const*, const dim_csr5_opts*, dim_csr5**) {
-    return fail(DIM_UNSUPPORTED, "not built yet");
-}
-
-dim_status dim_cg_batched(const dim_csr5* A, const double*, double*, const dim_cg_opts*, dim_cg_result*, void*) {
-    if (!A)
-        return fail(DIM_INVALID_ARGUMENT, "matrix handle is null");
-    return fail(DIM_UNSUPPORTED, "not built yet");
-}
-
 } // extern "C"

@@ -61,6 +61,10 @@ struct HostFormat {
 std::string build_format(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
                          const uint32_t* col_idx, const double* vals, uint32_t sigma, HostFormat& out);
 
+// Tile heights the autotuner times for a matrix with `rows` rows and `nnz`
+// stored elements.
+std::vector<uint32_t> sigma_candidates(uint32_t rows, uint64_t nnz);
+
 // Validates CSR structure (monotone row_ptr, row_ptr[0]==0, row_ptr[rows]==nnz,
 // column indices < cols). Returns an empty string or an error message.
 std::string validate_csr(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,

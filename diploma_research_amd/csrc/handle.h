@@ -21,11 +21,9 @@ struct dim_csr5 {
     // block-diagonal batches: system i owns rows [sys_row[i], sys_row[i+1]) and
     // tiles [sys_tile[i], sys_tile[i+1])
     uint32_t n_systems = 1;
-    std::vector<uint32_t> sys_row, sys_tile, sys_nnz_csr;
-    uint32_t* d_sys_row = nullptr;  // n_systems + 1
-    uint32_t* d_sys_tile = nullptr; // n_systems + 1
-    // CSR-order position of every stored element (for batched matrices the
-    // element order of the concatenated per-system CSR arrays)
+    std::vector<uint32_t> sys_row, sys_tile;
+    std::vector<uint64_t> sys_nnz_csr;
+    uint64_t nnz_logical = 0; // stored elements (the format's nnz counts padding inside a batch)
 };
 
 namespace dimgpu {
@@ -68,6 +66,9 @@ hipError_t upload(dim_csr5* h, T*& p, const std::vector<T>& v) {
 
 // Uploads a host format into a fresh handle (allocates scratch too).
 dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out);
+
+// One system's share of a batched matrix as a stand-alone format view.
+DevFormat system_view(const dim_csr5* A, uint32_t s);
 void free_handle(dim_csr5* h);
 
 } // namespace dimgpu

@@ -54,7 +54,11 @@ hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const vo
                               hipEvent_t e1, hipEvent_t e2, hipStream_t stream);
 
 // y = A^T x (x has rows entries, y has cols entries); y is zeroed first.
-hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, void* y, hipStream_t stream);
+hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, void* y, hipStream_t stream,
+                                 const int* done = nullptr);
+
+// Scatters CSR-order values into the tile layout (FP64, and FP32 if kept).
+hipError_t launch_set_values(const DevFormat& A, const double* vals_csr, hipStream_t stream);
 
 // out[e] = scale * u[row(e)] * v[col(e)] for every stored element e, in CSR order.
 hipError_t launch_sampled_outer(const DevFormat& A, double scale, const double* u, const double* v, double* out,

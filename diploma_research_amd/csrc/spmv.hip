@@ -29,7 +29,9 @@ using namespace detail;
 // ------------------------------------------------------ transpose SpMV ---
 
 template<typename V>
-__global__ void k_fill(V* y, size_t n, V value) {
+__global__ void k_fill(V* y, size_t n, V value, const int* done) {
+    if (done && *done)
+        return;
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * blockDim.x)
         y[i] = value;
@@ -47,6 +49,8 @@ __device__ __forceinline__ uint32_t element_row(uint32_t bits, int r, int y_off,
 // into y with hardware floating-point atomics.
 template<typename V, int SIGMA>
 __global__ __launch_bounds__(256) void k_csr5_transpose(TileParams p) {
+    if (p.done && *p.done)
+        return;
     const int lane = threadIdx.x & 63;
     const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= p.tile_end)
@@ -102,6 +106,26 @@ __global__ __launch_bounds__(256) void k_sampled_outer(TileParams p, double scal
             continue;
         const uint32_t row = element_row(bits, r, y_off, row0, dirty, tdo);
         out[csr] = scale * u[row] * v[p.col_idx[pos]];
+    }
+}
+
+// Tile-layout values from CSR-order values: the same permutation as the
+// conversion's transpose, run on the device.
+template<int SIGMA>
+__global__ __launch_bounds__(256) void k_set_values(TileParams p, const double* src, double* v64, float* v32) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= p.tile_end)
+        return;
+    const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
+#pragma unroll
+    for (int r = 0; r < SIGMA; ++r) {
+        const size_t pos = base + static_cast<size_t>(r) * 64 + lane;
+        const size_t csr = base + static_cast<size_t>(lane) * SIGMA + r;
+        const double a = csr < p.nnz ? src[csr] : 0.0;
+        v64[pos] = a;
+        if (v32)
+            v32[pos] = static_cast<float>(a);
     }
 }
 
@@ -218,17 +242,19 @@ hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const vo
     return hipGetLastError();
 }
 
-hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, void* y, hipStream_t stream) {
+hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, void* y, hipStream_t stream,
+                                 const int* done) {
     TileParams tp = make_tile_params(A, f32);
     tp.x = x;
     tp.y = y;
+    tp.done = done;
     const uint32_t fill_grid = 1024;
     if (f32)
         hipLaunchKernelGGL(k_fill<float>, dim3(fill_grid), dim3(256), 0, stream, static_cast<float*>(y),
-                           static_cast<size_t>(A.cols), 0.0f);
+                           static_cast<size_t>(A.cols), 0.0f, done);
     else
         hipLaunchKernelGGL(k_fill<double>, dim3(fill_grid), dim3(256), 0, stream, static_cast<double*>(y),
-                           static_cast<size_t>(A.cols), 0.0);
+                           static_cast<size_t>(A.cols), 0.0, done);
     if (A.tiles) {
         const uint32_t grid = (A.tiles + 3) / 4;
 #define DIM_T_F(S)                                                                                                    \
@@ -242,6 +268,23 @@ hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, vo
         }
 #undef DIM_T_F
 #undef DIM_T_D
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_set_values(const DevFormat& A, const double* vals_csr, hipStream_t stream) {
+    if (!A.tiles)
+        return hipSuccess;
+    TileParams tp = make_tile_params(A, false);
+    const uint32_t grid = (A.tiles + 3) / 4;
+    switch (A.sigma) {
+#define DIM_SV(S)                                                                                                     \
+    case S:                                                                                                           \
+        hipLaunchKernelGGL((k_set_values<S>), dim3(grid), dim3(256), 0, stream, tp, vals_csr, A.vals64, A.vals32);   \
+        break;
+        DIM_FOR_EACH_SIGMA(DIM_SV)
+#undef DIM_SV
+        default: break;
     }
     return hipGetLastError();
 }

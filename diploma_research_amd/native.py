@@ -25,6 +25,7 @@ DIM_F32 = 1
 DIM_CSR5_KEEP_F32 = 1
 DIM_CG_F64 = 0
 DIM_CG_F32_REFINED = 1
+DIM_CG_TRANSPOSE = 1
 
 # Every symbol include/dim_gpu.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -32,7 +33,7 @@ EXPORTS = [
     "dim_csr5_from_csr", "dim_csr5_from_csr_device", "dim_csr5_destroy", "dim_csr5_get_info",
     "dim_csr5_copy_format", "dim_csr5_spmv", "dim_csr5_spmv_timed", "dim_csr5_block_rows", "dim_csr5_spmv_block", "dim_csr5_spmm",
     "dim_csr5_spmv_transpose", "dim_csr5_sampled_outer", "dim_cg", "dim_cg_multi", "dim_csr5_from_csr_batched",
-    "dim_cg_batched", "dim_csr5_n_systems",
+    "dim_cg_batched", "dim_csr5_n_systems", "dim_csr5_set_values",
 ]
 
 
@@ -49,7 +50,8 @@ class Csr5Info(C.Structure):
 
 class CgOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("max_iters", C.c_uint32), ("precision", C.c_uint32),
-                ("check_every", C.c_uint32), ("refine_delta", C.c_double), ("history", C.c_void_p)]
+                ("check_every", C.c_uint32), ("refine_delta", C.c_double), ("history", C.c_void_p),
+                ("flags", C.c_uint32)]
 
 
 class CgResult(C.Structure):
@@ -97,6 +99,7 @@ def lib():
         "dim_csr5_from_csr_batched": ([u32, vp, vp, vp, vp, vp, P(Csr5Opts), P(vp)], i32),
         "dim_cg_batched": ([vp, vp, vp, P(CgOpts), P(CgResult), vp], i32),
         "dim_csr5_n_systems": ([vp], u32),
+        "dim_csr5_set_values": ([vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

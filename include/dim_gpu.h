@@ -147,7 +147,9 @@ typedef struct dim_cg_opts {
     uint32_t check_every;  /* iterations between host convergence checks (0 = 16) */
     double refine_delta;   /* F32_REFINED: FP64 residual replacement when ||r|| drops by this factor (0 = 0.1) */
     double* history;       /* optional HOST array of max_iters+1: ||r_i||/||b|| (recursive residual) */
+    uint32_t flags;        /* DIM_CG_TRANSPOSE: iterate with A^T (the adjoint solve A^T l = g); batched only */
 } dim_cg_opts;
+#define DIM_CG_TRANSPOSE 1u
 
 typedef struct dim_cg_result {
     uint32_t iterations;
@@ -171,15 +173,27 @@ dim_status dim_cg_multi(const dim_csr5* A, uint32_t k, const double* B, double* 
                         dim_cg_result* res, void* stream);
 
 /* Many independent SPD systems in one launch sequence: `batch` is a
- * block-diagonal matrix built with dim_csr5_from_csr_batched, B/X hold the
- * systems' vectors back to back (DEVICE, FP64). The whole iteration loop is
- * captured once in a hipGraph and replayed. res has n_systems entries. */
+ * block-diagonal matrix built with dim_csr5_from_csr_batched (host CSR arrays
+ * per system, each square; rows[i] rows and nnz[i] entries), B/X hold the
+ * systems' vectors back to back (DEVICE, FP64). Every system keeps its own
+ * step lengths and stop test; a batch of iterations is captured once in a
+ * hipGraph and replayed until every system has stopped. res has n_systems
+ * entries. With opts->flags & DIM_CG_TRANSPOSE the products use A^T (the
+ * adjoint solve). No reference counterpart (the reference solves one system
+ * per process). Padding elements of each system's last tile are stored as
+ * 0 * x[first row of the system], so x must be finite. */
 dim_status dim_csr5_from_csr_batched(uint32_t n_systems, const uint32_t* rows, const uint64_t* nnz,
                                      const uint32_t* const* row_ptr, const uint32_t* const* col_idx,
                                      const double* const* vals, const dim_csr5_opts* opts, dim_csr5** out);
 dim_status dim_cg_batched(const dim_csr5* batch, const double* B, double* X, const dim_cg_opts* opts,
                           dim_cg_result* res, void* stream);
 uint32_t dim_csr5_n_systems(const dim_csr5* A);
+
+/* Replaces the stored values (same sparsity pattern) from a DEVICE FP64 array
+ * in the CSR order the matrix was built from (for batched matrices: the
+ * systems' arrays back to back). Keeps the FP32 copy in step when present. No
+ * reference counterpart; used when differentiating with respect to A. */
+dim_status dim_csr5_set_values(dim_csr5* A, const double* vals, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,198 @@
+"""Batched CG (many uneven SPD systems in one launch sequence) and the
+hand-written adjoint behind torch.autograd: checked against the oracle's CG
+per system, dense FP64 solves, and central finite differences."""
+import numpy as np
+import pytest
+
+from gpu_util import to_dev
+from diploma_research_amd import Csr5, conjugate_gradient, matgen
+from diploma_research_amd.csr5 import conjugate_gradient_batched
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(sides, sigma_log, seed):
+    out = []
+    for i, s in enumerate(sides):
+        rp, ci, v = matgen.varcoef((s, s), sigma_log, seed=seed + i)
+        out.append((s * s, rp, ci, v))
+    return out
+
+
+def _dense(rp, ci, v, n):
+    D = np.zeros((n, n))
+    D[np.repeat(np.arange(n), np.diff(rp.astype(np.int64))), ci] = v
+    return D
+
+
+@pytest.mark.parametrize("sigma", [0, 5, 16])
+def test_batched_cg_matches_oracle_per_system(cuda, sigma):
+    sides = [11, 37, 5, 60, 23, 48]
+    systems = _systems(sides, 0.5, seed=1)
+    A = Csr5.from_csr_batched(systems, sigma=sigma)
+    assert A.n_systems == len(sides)
+    rng = np.random.default_rng(0)
+    bs = [rng.uniform(-1, 1, n) for n, *_ in systems]
+    X, res = conjugate_gradient_batched(A, to_dev(np.concatenate(bs)), rtol=1e-10, max_iters=5000)
+    X = X.cpu().numpy()
+    off = 0
+    for (n, rp, ci, v), b, r in zip(systems, bs, res):
+        oc = O.Csr5(n, n, rp, ci, v).cg(b, rtol=1e-10, maxit=5000)
+        assert r.converged and abs(r.iterations - oc["iterations"]) <= 2, (r.iterations, oc["iterations"])
+        assert r.relres_true <= 2e-10
+        np.testing.assert_allclose(X[off: off + n], oc["x"], rtol=1e-7, atol=1e-8)
+        off += n
+
+
+def test_batched_products_and_values(cuda):
+    import torch
+
+    systems = _systems([9, 31, 17], 0.5, seed=4)
+    A = Csr5.from_csr_batched(systems, sigma=8)
+    x = np.random.default_rng(1).uniform(-1, 1, sum(n for n, *_ in systems))
+    y = A.spmv(to_dev(x)).cpu().numpy()
+    yt = A.spmv_transpose(to_dev(x)).cpu().numpy()
+    off = 0
+    for n, rp, ci, v in systems:
+        ref = O.csr_spmv(rp, ci, v, x[off: off + n].copy())
+        np.testing.assert_allclose(y[off: off + n], ref, rtol=1e-13, atol=1e-14)
+        np.testing.assert_allclose(yt[off: off + n], ref, rtol=1e-12, atol=1e-13)  # symmetric systems
+        off += n
+    # replace the values: 2*A
+    vals = torch.from_numpy(np.concatenate([v for *_, v in systems]) * 2.0).cuda()
+    A.set_values(vals)
+    np.testing.assert_allclose(A.spmv(to_dev(x)).cpu().numpy(), 2 * y, rtol=1e-14, atol=1e-14)
+
+
+def test_transposed_solve_matches_forward_on_spd(cuda):
+    systems = _systems([25, 14], 0.5, seed=9)
+    A = Csr5.from_csr_batched(systems, sigma=8)
+    b = to_dev(np.random.default_rng(3).uniform(-1, 1, 25 * 25 + 14 * 14))
+    X1, r1 = conjugate_gradient_batched(A, b, rtol=1e-12)
+    X2, r2 = conjugate_gradient_batched(A, b, rtol=1e-12, transpose=True)
+    assert all(r.converged for r in r1 + r2)
+    np.testing.assert_allclose(X2.cpu().numpy(), X1.cpu().numpy(), rtol=1e-9, atol=1e-11)
+
+
+def test_batched_breakdown_is_reported(cuda):
+    from diploma_research_amd import DimError
+
+    n = 50
+    d = np.where(np.arange(n) % 3 == 0, -1.0, 2.0)
+    good = _systems([6], 0.5, seed=2)[0]
+    bad = (n, np.arange(n + 1, dtype=np.uint32), np.arange(n, dtype=np.uint32), d)
+    A = Csr5.from_csr_batched([good, bad], sigma=8)
+    with pytest.raises(DimError, match="breakdown"):
+        conjugate_gradient_batched(A, to_dev(np.ones(36 + n)), rtol=1e-10)
+
+
+# ------------------------------------------------------------- adjoint ---
+
+def _loss_dense(systems, vals_list, bs):
+    L = 0.0
+    for (n, rp, ci, _), v, b in zip(systems, vals_list, bs):
+        x = np.linalg.solve(_dense(rp, ci, v, n), b)
+        L += 0.5 * float(x @ x)
+    return L
+
+
+def test_adjoint_gradient_matches_dense_reference_and_finite_differences(cuda):
+    """L = 0.5 ||x||^2, x = A^-1 b over a batch of uneven 2-D variable-
+    coefficient systems. dL/db and dL/dvals from the adjoint (transpose CG +
+    sampled outer product) against (1) the dense FP64 reference gradient and
+    (2) central finite differences, h = 1e-6, on 64 random entries, perturbing
+    a_ij and a_ji together (the systems are symmetric, so the gradient of that
+    parametrisation is g_ij + g_ji)."""
+    import torch
+
+    from diploma_research_amd.autograd import solve
+
+    sides = [7, 12, 9, 15, 10, 13]
+    systems = _systems(sides, 0.5, seed=21)
+    A = Csr5.from_csr_batched(systems, sigma=8)
+    rng = np.random.default_rng(5)
+    bs = [rng.uniform(-1, 1, n) for n, *_ in systems]
+    vals_np = np.concatenate([v for *_, v in systems])
+    vals = torch.from_numpy(vals_np.copy()).cuda().requires_grad_(True)
+    b = torch.from_numpy(np.concatenate(bs)).cuda().requires_grad_(True)
+    x = solve(A, vals, b, rtol=1e-14, max_iters=20000)
+    L = 0.5 * (x * x).sum()
+    L.backward()
+    gv = vals.grad.cpu().numpy()
+    gb = b.grad.cpu().numpy()
+
+    # (1) dense FP64 reference gradient
+    off_v = off_r = 0
+    ref_gv = np.zeros_like(gv)
+    ref_gb = np.zeros_like(gb)
+    for (n, rp, ci, v), bb in zip(systems, bs):
+        D = _dense(rp, ci, v, n)
+        xs = np.linalg.solve(D, bb)
+        lam = np.linalg.solve(D.T, xs)
+        rows = np.repeat(np.arange(n), np.diff(rp.astype(np.int64)))
+        ref_gv[off_v: off_v + len(v)] = -lam[rows] * xs[ci]
+        ref_gb[off_r: off_r + n] = lam
+        off_v += len(v)
+        off_r += n
+    np.testing.assert_allclose(gb, ref_gb, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(gv, ref_gv, rtol=1e-9, atol=1e-12)
+
+    # (2) central finite differences on 64 random stored entries
+    h = 1e-6
+    vals_list = [v.copy() for *_, v in systems]
+    starts = np.concatenate([[0], np.cumsum([len(v) for v in vals_list])])
+    picks = rng.choice(len(vals_np), size=64, replace=False)
+    worst = 0.0
+    for e in picks:
+        s = int(np.searchsorted(starts, e, side="right") - 1)
+        n, rp, ci, v = systems[s]
+        k = e - starts[s]
+        i = int(np.searchsorted(rp.astype(np.int64), k, side="right") - 1)
+        j = int(ci[k])
+        mirror = [k]
+        if i != j:
+            row_j = slice(int(rp[j]), int(rp[j + 1]))
+            kk = int(rp[j]) + int(np.nonzero(ci[row_j] == i)[0][0])
+            mirror.append(kk)
+        # only system s depends on the entry; L(+h) - L(-h) is evaluated as
+        # 0.5 (x+ - x-).(x+ + x-) so that the difference is not lost to the
+        # rounding of L itself
+        xs_pm = []
+        for sign in (+1, -1):
+            vs = vals_list[s].copy()
+            for m in mirror:
+                vs[m] += sign * h
+            xs_pm.append(np.linalg.solve(_dense(rp, ci, vs, n), bs[s]))
+        fd = 0.5 * float((xs_pm[0] - xs_pm[1]) @ (xs_pm[0] + xs_pm[1])) / (2 * h)
+        an = sum(gv[starts[s] + m] for m in mirror)
+        rel = abs(fd - an) / max(abs(an), 1e-12)
+        worst = max(worst, rel)
+    assert worst <= 1e-6, worst
+
+
+def test_adjoint_gradient_wrt_b_by_finite_differences_on_gpu(cuda):
+    """FD through the GPU solver itself (every evaluation a batched CG)."""
+    import torch
+
+    from diploma_research_amd.autograd import solve
+
+    systems = _systems([8, 11], 0.5, seed=31)
+    A = Csr5.from_csr_batched(systems, sigma=8)
+    vals = torch.from_numpy(np.concatenate([v for *_, v in systems])).cuda()
+    n = 64 + 121
+    b0 = np.random.default_rng(8).uniform(-1, 1, n)
+    b = torch.from_numpy(b0.copy()).cuda().requires_grad_(True)
+    x = solve(A, vals, b, rtol=1e-15, max_iters=20000)
+    (0.5 * (x * x).sum()).backward()
+    g = b.grad.cpu().numpy()
+    h = 1e-6
+    for idx in np.random.default_rng(2).choice(n, size=8, replace=False):
+        xs_pm = []
+        for sgn in (+1, -1):
+            bb = b0.copy()
+            bb[idx] += sgn * h
+            xx, _ = conjugate_gradient_batched(A, to_dev(bb), rtol=1e-15, max_iters=20000)
+            xs_pm.append(xx.cpu().numpy())
+        fd = 0.5 * float((xs_pm[0] - xs_pm[1]) @ (xs_pm[0] + xs_pm[1])) / (2 * h)
+        assert abs(fd - g[idx]) <= 1e-6 * max(abs(g[idx]), 1e-3), (idx, fd, g[idx])

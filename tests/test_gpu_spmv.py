@@ -241,3 +241,36 @@ def test_powerlaw_4m_full_size(cuda):
     A = Csr5.from_csr((n, n, rp, ci, v), sigma=16)
     y = A.spmv(to_dev(x)).cpu().numpy()
     assert_spmv_close(y, O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("sigma", [4, 7, 16, 32])
+def test_gpu_conversion_equals_host_conversion(cuda, case, sigma):
+    """CSR -> CSR5 on the GPU (from device CSR arrays) builds exactly the
+    format of the host builder, so products are bitwise identical."""
+    import torch
+
+    c = CASES[case]
+    rp, ci, v = matgen.random_csr(c["rows"], c["cols"], seed=200 + case, max_len=c["max_len"],
+                                  empty_frac=c["empty_frac"], long_rows=c["long_rows"])
+    H = Csr5.from_csr((c["rows"], c["cols"], rp, ci, v), sigma=sigma, keep_f32=True)
+    D = Csr5.from_csr_device(c["rows"], c["cols"], to_dev(rp.astype(np.int32)), to_dev(ci.astype(np.int32)),
+                             to_dev(v), sigma=sigma, keep_f32=True)
+    fh, fd = H.format_arrays(), D.format_arrays()
+    for key in fh:
+        np.testing.assert_array_equal(fd[key], fh[key], err_msg=key)
+    assert D.info.n_dirty_tiles == H.info.n_dirty_tiles and D.info.n_empty_rows == H.info.n_empty_rows
+    x = to_dev(np.random.default_rng(case).uniform(-1, 1, c["cols"]))
+    assert torch.equal(D.spmv(x), H.spmv(x))
+    assert torch.equal(D.spmv(x.float()), H.spmv(x.float()))
+
+
+def test_gpu_conversion_rejects_bad_csr(cuda):
+    rp = np.array([0, 2, 1, 3], np.int32)
+    ci = np.array([0, 1, 2], np.int32)
+    with pytest.raises(ValueError, match="non-decreasing"):
+        Csr5.from_csr_device(3, 3, to_dev(rp), to_dev(ci), to_dev(np.ones(3)))
+    rp = np.array([0, 1, 2, 3], np.int32)
+    ci = np.array([0, 1, 7], np.int32)
+    with pytest.raises(ValueError, match="out of range"):
+        Csr5.from_csr_device(3, 3, to_dev(rp), to_dev(ci), to_dev(np.ones(3)))
