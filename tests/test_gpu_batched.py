@@ -101,9 +101,9 @@ def test_adjoint_gradient_matches_dense_reference_and_finite_differences(cuda):
     """L = 0.5 ||x||^2, x = A^-1 b over a batch of uneven 2-D variable-
     coefficient systems. dL/db and dL/dvals from the adjoint (transpose CG +
     sampled outer product) against (1) the dense FP64 reference gradient and
-    (2) central finite differences, h = 1e-6, on 64 random entries, perturbing
-    a_ij and a_ji together (the systems are symmetric, so the gradient of that
-    parametrisation is g_ij + g_ji)."""
+    (2) fourth-order central finite differences, h = 1e-3, on 64 random
+    entries, perturbing a_ij and a_ji together (the systems are symmetric, so
+    the gradient of that parametrisation is g_ij + g_ji)."""
     import torch
 
     from diploma_research_amd.autograd import solve
@@ -138,8 +138,9 @@ def test_adjoint_gradient_matches_dense_reference_and_finite_differences(cuda):
     np.testing.assert_allclose(gb, ref_gb, rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(gv, ref_gv, rtol=1e-9, atol=1e-12)
 
-    # (2) central finite differences on 64 random stored entries
-    h = 1e-6
+    # (2) fourth-order central finite differences, h = 1e-3, on 64 random
+    # stored entries
+    h = 1e-3
     vals_list = [v.copy() for *_, v in systems]
     starts = np.concatenate([[0], np.cumsum([len(v) for v in vals_list])])
     picks = rng.choice(len(vals_np), size=64, replace=False)
@@ -155,16 +156,20 @@ def test_adjoint_gradient_matches_dense_reference_and_finite_differences(cuda):
             row_j = slice(int(rp[j]), int(rp[j + 1]))
             kk = int(rp[j]) + int(np.nonzero(ci[row_j] == i)[0][0])
             mirror.append(kk)
-        # only system s depends on the entry; L(+h) - L(-h) is evaluated as
-        # 0.5 (x+ - x-).(x+ + x-) so that the difference is not lost to the
-        # rounding of L itself
-        xs_pm = []
-        for sign in (+1, -1):
+
+        def x_at(d):
             vs = vals_list[s].copy()
             for m in mirror:
-                vs[m] += sign * h
-            xs_pm.append(np.linalg.solve(_dense(rp, ci, vs, n), bs[s]))
-        fd = 0.5 * float((xs_pm[0] - xs_pm[1]) @ (xs_pm[0] + xs_pm[1])) / (2 * h)
+                vs[m] += d
+            return np.linalg.solve(_dense(rp, ci, vs, n), bs[s])
+
+        # only system s depends on the entry; L(a) - L(b) is evaluated as
+        # 0.5 (x_a - x_b).(x_a + x_b) so that the difference is not lost to the
+        # rounding of L itself
+        def dl(xa, xb):
+            return 0.5 * float((xa - xb) @ (xa + xb))
+
+        fd = (8 * dl(x_at(h), x_at(-h)) - dl(x_at(2 * h), x_at(-2 * h))) / (12 * h)
         an = sum(gv[starts[s] + m] for m in mirror)
         rel = abs(fd - an) / max(abs(an), 1e-12)
         worst = max(worst, rel)

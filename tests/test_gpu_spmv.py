@@ -208,7 +208,9 @@ def test_row_block_shares(cuda, sigma):
 def test_autotuned_sigma(cuda):
     rp, ci, v = matgen.poisson2d(128)
     A = Csr5.from_csr((16384, 16384, rp, ci, v), sigma=0)
-    assert A.sigma in (8, 16, 24, 32)
+    # the tuner picks among the supported tile heights near 1-4x the mean row
+    # length (5 here) plus 4, 8 and 16
+    assert A.sigma in (4, 5, 6, 7, 8, 10, 12, 14, 16, 20)
     x = np.random.default_rng(0).uniform(-1, 1, 16384)
     assert_spmv_close(A.spmv(to_dev(x)).cpu().numpy(), O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
 
