@@ -66,6 +66,8 @@ dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out) {
         d.bit_flag = p;
     }
     DIM_HIP_TRY(upload(h.get(), d.tile_flags, f.tile_flags));
+    h->binfo.assign(f.tile_flags);
+    d.binfo = &h->binfo;
     DIM_HIP_TRY(upload(h.get(), d.tile_desc_offset_ptr, f.tile_desc_offset_ptr));
     DIM_HIP_TRY(upload(h.get(), d.tile_desc_offset, f.tile_desc_offset));
     DIM_HIP_TRY(upload(h.get(), d.col_idx, f.col_idx));
@@ -130,6 +132,7 @@ DevFormat system_view(const dim_csr5* A, uint32_t s) {
     view.tile_ptr = A->f.tile_ptr + t0;
     view.bit_flag = static_cast<char*>(A->f.bit_flag) + size_t{t0} * kOmega * (A->f.sigma <= 16 ? 2 : 4);
     view.tile_flags = A->f.tile_flags + t0;
+    view.binfo_base = A->f.binfo_base + t0;
     view.tile_desc_offset_ptr = A->f.tile_desc_offset_ptr + t0;
     view.col_idx = A->f.col_idx + t0 * tile;
     view.vals64 = A->f.vals64 + t0 * tile;

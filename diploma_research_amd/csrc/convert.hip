@@ -312,6 +312,11 @@ dim_status convert(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* r
     DIM_HIP_TRY(dev_alloc(h.get(), sc.dot_chunk, Tm * kMaxK));
     DIM_HIP_TRY(dev_alloc(h.get(), sc.dot_blk, ((Tm + 255) / 256) * kMaxK));
     DIM_HIP_TRY(hipStreamSynchronize(s));
+    // host copy of the tile flags for the launch planner
+    std::vector<uint8_t> hflags(size_t{T} + 1);
+    DIM_HIP_TRY(hipMemcpy(hflags.data(), d.tile_flags, size_t{T} + 1, hipMemcpyDeviceToHost));
+    h->binfo.assign(std::move(hflags));
+    d.binfo = &h->binfo;
     // dirty-tile count for get_info
     if (ntdo) {
         std::vector<uint32_t> nsd(T);

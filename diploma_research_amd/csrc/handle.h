@@ -24,6 +24,7 @@ struct dim_csr5 {
     std::vector<uint32_t> sys_row, sys_tile;
     std::vector<uint64_t> sys_nnz_csr;
     uint64_t nnz_logical = 0; // stored elements (the format's nnz counts padding inside a batch)
+    dimgpu::BoundaryInfo binfo; // host copy of the tile flags (f.binfo points here)
 };
 
 namespace dimgpu {

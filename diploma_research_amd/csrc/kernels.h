@@ -4,8 +4,32 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <vector>
 
 namespace dimgpu {
+
+// Host copy of the tile flags, used to decide whether a product needs the
+// fix-up kernel: with chunks of `tpw` tiles, a row that crosses into a chunk
+// is finished by the earlier chunk (look-ahead) unless it began before that
+// chunk's last tile or runs past the later chunk's first tile.
+class BoundaryInfo {
+  public:
+    void assign(std::vector<uint8_t> flags) {
+        std::lock_guard<std::mutex> g(m_);
+        flags_ = std::move(flags);
+        memo_valid_ = false;
+    }
+    // tiles [tb, te) of the flag array, chunks of tpw tiles
+    bool needs_fixup(uint32_t tb, uint32_t te, uint32_t tpw) const;
+
+  private:
+    std::vector<uint8_t> flags_;
+    mutable std::mutex m_;
+    mutable bool memo_valid_ = false;
+    mutable uint32_t memo_tb_ = 0, memo_te_ = 0, memo_tpw_ = 0;
+    mutable bool memo_need_ = true;
+};
 
 // Device pointers of one matrix (see HostFormat for the layout).
 struct DevFormat {
@@ -22,6 +46,10 @@ struct DevFormat {
     uint32_t* empty_rows = nullptr;
     uint32_t n_empty = 0;
     bool any_dirty = false;
+    // host-side tile flags of the whole matrix; tile t of this view is tile
+    // binfo_base + t there
+    const BoundaryInfo* binfo = nullptr;
+    uint32_t binfo_base = 0;
 };
 
 // Per-matrix scratch: tile carries and reduction partials (the GPU

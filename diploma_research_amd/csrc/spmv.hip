@@ -150,6 +150,28 @@ TileParams make_tile_params(const DevFormat& A, bool f32) {
 
 using namespace detail;
 
+bool BoundaryInfo::needs_fixup(uint32_t tb, uint32_t te, uint32_t tpw) const {
+    std::lock_guard<std::mutex> g(m_);
+    if (memo_valid_ && memo_tb_ == tb && memo_te_ == te && memo_tpw_ == tpw)
+        return memo_need_;
+    bool need = te > flags_.size() || tpw == 0;
+    if (!need && tb < te) {
+        // the first chunk keeps a row that began before the range in its head slot
+        need = !(flags_[tb] & kStartsRow);
+        for (uint64_t t = uint64_t{tb} + tpw; !need && t < te; t += tpw) {
+            const uint8_t f = flags_[t];
+            const bool looked = (f & kMultiSeg) && (flags_[t - 1] & (kStartsRow | kMultiSeg));
+            need = !(f & kStartsRow) && !looked;
+        }
+    }
+    memo_valid_ = true;
+    memo_tb_ = tb;
+    memo_te_ = te;
+    memo_tpw_ = tpw;
+    memo_need_ = need;
+    return need;
+}
+
 int detail::device_cus() {
     static int cus[64] = {};
     int dev = 0;
@@ -181,6 +203,9 @@ void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x,
     tp.tile_end = tile_end;
     tp.row_base = row_base;
     tp.k = k;
+    tp.empty_rows = A.empty_rows;
+    tp.empty_begin = empty_begin;
+    tp.empty_end = empty_end;
     fp = FixParams{};
     fp.tile_ptr = A.tile_ptr;
     fp.empty_rows = A.empty_rows;

@@ -58,6 +58,7 @@ constexpr int kDppRowShr4 = 0x114;
 constexpr int kDppRowShr8 = 0x118;
 constexpr int kDppRowBcast15 = 0x142;
 constexpr int kDppRowBcast31 = 0x143;
+constexpr int kDppWaveShr1 = 0x138; // lane i takes lane i-1 (lane 0 takes 0)
 
 template<int CTRL, int ROW_MASK>
 __device__ __forceinline__ int dpp_int(int v) {
@@ -130,7 +131,8 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
+template<typename V>
+__device__ __forceinline__ V wave_sum(V v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1)
         v += __shfl_xor(v, d, 64);
@@ -149,6 +151,7 @@ __device__ __forceinline__ V ldg_stream(const V* p) {
 constexpr uint8_t kChunkStarts = 1; // the chunk's first element starts a row
 constexpr uint8_t kChunkHead = 2;   // head slot holds the closing part of a row from earlier chunks
 constexpr uint8_t kChunkTail = 4;   // tail slot holds the open part of a row that continues
+constexpr uint8_t kChunkRowStart = 8; // a row starts somewhere inside the chunk
 
 struct TileParams {
     const uint32_t* tile_ptr;
@@ -165,10 +168,13 @@ struct TileParams {
     uint8_t* chunk_flags;
     double* dot_chunk;
     const int* done;
+    const uint32_t* empty_rows;
     uint64_t nnz;
     uint32_t tile_begin, tile_end, row_base, last_tile;
     uint32_t tpw, nchunks;
+    uint32_t empty_begin, empty_end; // empty rows written by this kernel
     int k;
+    int lookahead;
 };
 
 template<typename V, int SIGMA, int KMAX, bool DOT>
@@ -199,39 +205,104 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         carry[j] = V(0);
         dacc[j] = 0.0;
     }
-    const bool chunk_starts = p.tile_flags[t0] & kStartsRow;
+    const uint8_t tf0 = p.tile_flags[t0];
+    const bool chunk_starts = tf0 & kStartsRow;
     bool carry_is_head = !chunk_starts;
     uint8_t cflags = chunk_starts ? kChunkStarts : 0;
+    bool row_start_seen = false;
+
+    // Look-ahead (single-vector products without a fused dot): a row that
+    // starts in the last tile of a chunk and ends in the first tile of the
+    // next chunk is finished by the earlier chunk, which reads the few
+    // elements of that row in the next tile; the later chunk drops its part.
+    // Other rows that cross chunks go through the head/tail slots and the
+    // fix-up kernel. Both chunks decide from the same two tile flags.
+    constexpr bool LOOK = KMAX == 1 && !DOT;
+    const bool look = LOOK && p.lookahead;
+    bool drop_head = false;
+    if (look && !chunk_starts && t0 > p.tile_begin && (tf0 & kMultiSeg))
+        drop_head = (p.tile_flags[t0 - 1] & (kStartsRow | kMultiSeg)) != 0;
+
+    // Single-vector products keep the lane's whole column in registers so
+    // that all its loads are in flight at once. For short tiles they also run
+    // a two-stage pipeline over the chunk: the sigma values, column indices,
+    // bit flags and tile metadata of tile t+1 are in flight while the x
+    // gathers and the scans of tile t run, so the stream of matrix bytes does
+    // not stop while a wavefront works on a tile (taller tiles would lose more
+    // to the lower occupancy than they gain). Multi-RHS products stream the
+    // column step by step (their k-wide x rows dominate).
+    constexpr bool REG = KMAX == 1;
+    constexpr bool PIPE = REG && SIGMA <= 8;
+    constexpr int NREG = REG ? SIGMA : 1;
+    const V* __restrict__ vals_all = static_cast<const V*>(p.vals);
+    const W* __restrict__ bits_all = static_cast<const W*>(p.bit_flag);
+
+    struct Vec {
+        V a[KMAX];
+    };
+    struct Stage {
+        V v[NREG];
+        uint32_t col[NREG];
+        W bits;
+        uint32_t tile_ptr, tdo_at;
+        uint8_t tf, tf_next;
+    };
+    auto fetch = [&](uint32_t t, Stage& s) {
+        const size_t base = static_cast<size_t>(t) * (64 * SIGMA) + lane;
+        if constexpr (REG) {
+#pragma unroll
+            for (int r = 0; r < SIGMA; ++r) {
+                s.v[r] = ldg_stream(vals_all + base + r * 64);
+                s.col[r] = ldg_stream(p.col_idx + base + r * 64);
+            }
+        }
+        s.bits = bits_all[static_cast<size_t>(t) * 64 + lane];
+        // tile_flags and tile_desc_offset_ptr hold T + 1 entries
+        s.tile_ptr = p.tile_ptr[t];
+        s.tdo_at = p.tdo_ptr[t];
+        s.tf = p.tile_flags[t];
+        s.tf_next = p.tile_flags[t + 1];
+    };
+    Stage nx;
+    if constexpr (PIPE)
+        fetch(t0, nx);
 
     for (uint32_t t = t0; t < t1; ++t) {
         const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
-        const V* vals = static_cast<const V*>(p.vals) + base + lane;
+        const V* vals = vals_all + base + lane;
         const uint32_t* cols = p.col_idx + base + lane;
 
-        // Single-vector products keep the lane's whole column in registers so
-        // that all 2*sigma loads are in flight at once; multi-RHS products
-        // stream the column step by step (their k-wide x rows dominate).
-        constexpr int NREG = KMAX == 1 ? SIGMA : 1;
-        V v[NREG];
-        uint32_t cidx[NREG];
-        if constexpr (KMAX == 1) {
+        Stage cu;
+        V xg[NREG];
+        if constexpr (PIPE) {
+            cu = nx;
 #pragma unroll
-            for (int r = 0; r < SIGMA; ++r) {
-                v[r] = ldg_stream(vals + r * 64);
-                cidx[r] = ldg_stream(cols + r * 64);
+            for (int r = 0; r < SIGMA; ++r)
+                xg[r] = x[cu.col[r]];
+            // keep the next tile's loads behind this tile's gathers: loads
+            // return in order, so waiting for the gathers then leaves the next
+            // tile's loads in flight
+            __builtin_amdgcn_sched_barrier(0);
+            fetch(t + 1 < t1 ? t + 1 : t, nx);
+        } else {
+            fetch(t, cu);
+            if constexpr (REG) {
+#pragma unroll
+                for (int r = 0; r < SIGMA; ++r)
+                    xg[r] = x[cu.col[r]];
             }
         }
 
-        const uint32_t bits =
-          static_cast<uint32_t>(static_cast<const W*>(p.bit_flag)[static_cast<size_t>(t) * 64 + lane]) |
-          (lane == 0 ? 1u : 0u);
-        const uint8_t tf = p.tile_flags[t];
+        const uint32_t bits = static_cast<uint32_t>(cu.bits) | (lane == 0 ? 1u : 0u);
+        const uint32_t tf = __builtin_amdgcn_readfirstlane(cu.tf);
         const bool starts_row = tf & kStartsRow;
-        const uint32_t tpw = p.tile_ptr[t];
+        const uint32_t tpw = __builtin_amdgcn_readfirstlane(cu.tile_ptr);
         const bool dirty = (tpw & kDirty) != 0;
         const uint32_t row0 = tpw & ~kDirty;
-        const uint32_t* tdo = dirty ? p.tdo + p.tdo_ptr[t] : nullptr;
-        const bool complete = (t + 1 == p.tile_end) || (p.tile_flags[t + 1] & kStartsRow);
+        const uint32_t tf_next = __builtin_amdgcn_readfirstlane(cu.tf_next);
+        const bool complete = (t + 1 == p.tile_end) || (tf_next & kStartsRow);
+        const bool has_row_start = (tf & (kStartsRow | kMultiSeg)) != 0;
+        row_start_seen |= has_row_start;
 
         // padded elements of the last tile contribute exactly zero
         uint32_t pad = 0;
@@ -248,181 +319,267 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         const int nseg = lane_value(incl, 63);
         const uint64_t started = __ballot(nb > 0);
 
-        auto write_row = [&](uint32_t row, const V* val) {
-            V* yr = y + static_cast<size_t>(row - p.row_base) * k;
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                if (j < k)
-                    yr[j] = val[j];
-            if constexpr (DOT) {
-                const V* pr = x + static_cast<size_t>(row) * k;
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    if (j < k)
-                        dacc[j] += static_cast<double>(pr[j]) * static_cast<double>(val[j]);
-            }
-        };
-        // segment 0 of a tile that does not start a row closes the carried row
-        auto close_carried = [&](const V* val) {
-            V tot[KMAX];
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                tot[j] = carry[j] + val[j];
-            if (carry_is_head) {
-                V* h = static_cast<V*>(p.head) + static_cast<size_t>(c) * k;
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    if (j < k)
-                        h[j] = tot[j];
-            } else {
-                write_row(row0, tot);
-            }
-        };
-        auto emit = [&](int s, const V* val) {
-            if (s == 0 && !starts_row)
-                close_carried(val);
-            else
-                write_row(row0 + (dirty ? tdo[s] : static_cast<uint32_t>(s)), val);
-        };
-
-        V cur[KMAX], red[KMAX];
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-            cur[j] = V(0);
-            red[j] = V(0);
-        }
-        bool has = false;
-        int seg = y_off - 1;
-        auto step = [&](int r, V vr, uint32_t cr) {
-            if ((bits >> r) & 1u) {
-                if (has)
-                    emit(seg, cur);
+        // The rest of the tile is instantiated twice: tiles without empty rows
+        // map segment s to row row0 + s with no memory access, so no load (and
+        // no wait for loads) sits between the gathers and the stores.
+        auto process = [&](auto dirty_tag) {
+            constexpr bool DIRTY = decltype(dirty_tag)::value;
+            const uint32_t* tdo = DIRTY ? p.tdo + __builtin_amdgcn_readfirstlane(cu.tdo_at) : nullptr;
+            auto row_of = [&](int s) -> uint32_t {
+                if constexpr (DIRTY)
+                    return row0 + tdo[s];
                 else
+                    return row0 + static_cast<uint32_t>(s);
+            };
+            auto write_row = [&](uint32_t row, const Vec& val) {
+                V* yr = y + static_cast<size_t>(row - p.row_base) * k;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    if (j < k)
+                        yr[j] = val.a[j];
+                if constexpr (DOT) {
+                    const V* pr = x + static_cast<size_t>(row) * k;
 #pragma unroll
                     for (int j = 0; j < KMAX; ++j)
-                        red[j] = cur[j];
-                has = true;
-                ++seg;
+                        if (j < k)
+                            dacc[j] += static_cast<double>(pr[j]) * static_cast<double>(val.a[j]);
+                }
+            };
+            // segment 0 of a tile that does not start a row closes the carried row
+            auto close_carried = [&](const Vec& val) {
+                Vec tot;
 #pragma unroll
                 for (int j = 0; j < KMAX; ++j)
-                    cur[j] = V(0);
+                    tot.a[j] = carry[j] + val.a[j];
+                if (carry_is_head) {
+                    if (drop_head)
+                        return; // the previous chunk writes this row
+                    V* h = static_cast<V*>(p.head) + static_cast<size_t>(c) * k;
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j)
+                        if (j < k)
+                            h[j] = tot.a[j];
+                } else {
+                    write_row(row0, tot);
+                }
+            };
+            // Segment 0 of a tile that does not start a row continues the
+            // carried row. The lane that closes it only keeps its partial;
+            // one lane adds it to the carry after the scan, so that the head
+            // slot and y are written from wave-uniform code only.
+            bool own0 = false;
+            Vec seg0;
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j)
+                seg0.a[j] = V(0);
+            auto emit = [&](int s, const Vec& val) {
+                if (s == 0 && !starts_row) {
+                    own0 = true;
+                    seg0 = val;
+                } else {
+                    write_row(row_of(s), val);
+                }
+            };
+
+            Vec cur, red;
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j) {
+                cur.a[j] = V(0);
+                red.a[j] = V(0);
             }
-            const V* xr = x + static_cast<size_t>(cr) * k;
-            const bool pz = (pad >> r) & 1u;
+            bool has = false;
+            int seg = y_off - 1;
+            auto step = [&](int r, V vr, uint32_t cr) {
+                if ((bits >> r) & 1u) {
+                    if (has)
+                        emit(seg, cur);
+                    else
+                        red = cur;
+                    has = true;
+                    ++seg;
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j)
+                        cur.a[j] = V(0);
+                }
+                const bool pz = (pad >> r) & 1u;
+                if constexpr (REG) {
+                    cur.a[0] = fma(vr, pz ? V(0) : xg[r], cur.a[0]);
+                } else {
+                    const V* xr = x + static_cast<size_t>(cr) * k;
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j) {
+                        if (j < k) {
+                            V xv = xr[j];
+                            if (pz)
+                                xv = V(0);
+                            cur.a[j] = fma(vr, xv, cur.a[j]);
+                        }
+                    }
+                }
+            };
+            if constexpr (REG) {
+#pragma unroll
+                for (int r = 0; r < SIGMA; ++r)
+                    step(r, cu.v[r], 0u);
+            } else {
+#pragma unroll 1
+                for (int r = 0; r < SIGMA; ++r)
+                    step(r, ldg_stream(vals + r * 64), ldg_stream(cols + r * 64));
+            }
+
+            V blue[KMAX];
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j) {
+                if (has) {
+                    blue[j] = cur.a[j];
+                } else {
+                    red.a[j] = cur.a[j];
+                    blue[j] = V(0);
+                }
+            }
+
+            // A lane in which a row starts closes the row left open by the
+            // previous such lane: that lane's trailing (blue) partial plus the
+            // leading (red) partials of the lanes in between, summed by a
+            // segmented inclusive scan (a new segment begins after every lane
+            // in which a row starts). When every lane starts a row (rows no
+            // longer than sigma, the common case) the scan is the identity and
+            // the previous such lane is the lane below: one DPP shift.
+            const bool all_lanes = started == ~0ull;
+            V A[KMAX], bc[KMAX];
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j)
+                A[j] = red.a[j];
+            if (all_lanes) {
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    bc[j] = (j < k) ? dpp_val<kDppWaveShr1, 0xf>(blue[j]) : V(0);
+            } else {
+                int f = (lane == 0) ? 1 : static_cast<int>((started >> (lane - 1)) & 1ull);
+                wave_seg_scan<V, KMAX>(A, f, k);
+                const uint64_t below = lane ? (started & ((1ull << lane) - 1ull)) : 0ull;
+                const int cprev = below ? 63 - __clzll(below) : 0;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    bc[j] = __shfl(blue[j], cprev, 64);
+            }
+            if (lane > 0 && has) {
+                Vec tot;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    tot.a[j] = bc[j] + A[j];
+                emit(y_off - 1, tot);
+            }
+
+            if (!starts_row && nseg > 1) {
+                // exactly one lane closed segment 0
+                const int o = __ffsll(static_cast<long long>(__ballot(own0))) - 1;
+                Vec s0;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    s0.a[j] = (j < k) ? lane_value(seg0.a[j], o) : V(0);
+                if (lane == 0)
+                    close_carried(s0);
+            }
+
+            // The tile's last segment, known to every lane.
+            const int L = all_lanes ? 63 : 63 - __clzll(started);
+            Vec tv;
 #pragma unroll
             for (int j = 0; j < KMAX; ++j) {
                 if (j < k) {
-                    V xv = xr[j];
-                    if (pz)
-                        xv = V(0);
-                    cur[j] = fma(vr, xv, cur[j]);
+                    const V bl = lane_value(blue[j], L);
+                    const V a63 = lane_value(A[j], 63);
+                    tv.a[j] = (L == 63) ? bl : bl + a63;
+                } else {
+                    tv.a[j] = V(0);
+                }
+            }
+            if (nseg == 1 && !starts_row) {
+                // the whole tile continues the carried row
+                if (complete) {
+                    if (lane == 0)
+                        close_carried(tv);
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j)
+                        carry[j] = V(0);
+                    carry_is_head = false;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j)
+                        carry[j] = carry[j] + tv.a[j];
+                }
+            } else {
+                if (!starts_row)
+                    carry_is_head = false; // segment 0 closed the carried row above
+                if (complete) {
+                    if (lane == 63)
+                        emit(nseg - 1, tv);
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j)
+                        carry[j] = V(0);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j)
+                        carry[j] = tv.a[j];
                 }
             }
         };
-        if constexpr (KMAX == 1) {
-#pragma unroll
-            for (int r = 0; r < SIGMA; ++r)
-                step(r, v[r], cidx[r]);
-        } else {
-#pragma unroll 1
-            for (int r = 0; r < SIGMA; ++r)
-                step(r, ldg_stream(vals + r * 64), ldg_stream(cols + r * 64));
-        }
+        if (dirty)
+            process(std::true_type{});
+        else
+            process(std::false_type{});
 
-        V blue[KMAX];
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-            if (has) {
-                blue[j] = cur[j];
-            } else {
-                red[j] = cur[j];
-                blue[j] = V(0);
-            }
-        }
-
-        // Segmented inclusive scan of the lanes' leading (red) partials; a new
-        // segment begins after every lane in which a row starts.
-        int f = (lane == 0) ? 1 : static_cast<int>((started >> (lane - 1)) & 1ull);
-        V A[KMAX];
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j)
-            A[j] = red[j];
-        wave_seg_scan<V, KMAX>(A, f, k);
-
-        // A lane in which a row starts closes the row left open by the
-        // previous such lane: its trailing (blue) partial plus the scanned reds.
-        const uint64_t below = lane ? (started & ((1ull << lane) - 1ull)) : 0ull;
-        const int cprev = below ? 63 - __clzll(below) : 0;
-        V bc[KMAX];
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j)
-            bc[j] = __shfl(blue[j], cprev, 64);
-        if (lane > 0 && has) {
-            V tot[KMAX];
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                tot[j] = bc[j] + A[j];
-            emit(y_off - 1, tot);
-        }
-
-        // The tile's last segment, known to every lane.
-        const int L = 63 - __clzll(started);
-        V tv[KMAX];
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-            if (j < k) {
-                const V bl = lane_value(blue[j], L);
-                const V a63 = lane_value(A[j], 63);
-                tv[j] = (L == 63) ? bl : bl + a63;
-            } else {
-                tv[j] = V(0);
-            }
-        }
-        if (nseg == 1 && !starts_row) {
-            // the whole tile continues the carried row
-            if (complete) {
-                if (lane == 0)
-                    close_carried(tv);
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    carry[j] = V(0);
-                carry_is_head = false;
-            } else {
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    carry[j] = carry[j] + tv[j];
-            }
-        } else {
-            if (!starts_row)
-                carry_is_head = false; // segment 0 closed the carried row above
-            if (complete) {
-                if (lane == 63)
-                    emit(nseg - 1, tv);
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    carry[j] = V(0);
-            } else {
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    carry[j] = tv[j];
-            }
-        }
         if (t + 1 == t1 && !complete) {
             // the chunk ends inside a row
-            cflags |= kChunkTail;
-            if (lane == 0) {
-                V* tl = static_cast<V*>(p.tail) + static_cast<size_t>(c) * k;
+            if (look && has_row_start && (tf_next & kMultiSeg)) {
+                // the row started in this tile and ends inside tile t1: add
+                // the elements of tile t1 before its first row start
+                if constexpr (LOOK) {
+                    const W wb = bits_all[static_cast<size_t>(t1) * 64 + lane];
+                    const uint64_t any = __ballot(wb != 0);
+                    const int ls = __ffsll(static_cast<long long>(any)) - 1;
+                    const uint32_t bl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wb), ls));
+                    const int rs = __ffs(static_cast<int>(bl)) - 1;
+                    constexpr uint32_t full = SIGMA >= 32 ? ~0u : ((1u << SIGMA) - 1u);
+                    const uint32_t m = lane < ls ? full : (lane == ls ? ((1u << rs) - 1u) : 0u);
+                    const size_t b1 = static_cast<size_t>(t1) * (64 * SIGMA) + lane;
+                    V part = V(0);
+                    if (m) {
 #pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    if (j < k)
-                        tl[j] = carry[j];
+                        for (int r = 0; r < SIGMA; ++r)
+                            if ((m >> r) & 1u)
+                                part = fma(vals_all[b1 + r * 64], x[p.col_idx[b1 + r * 64]], part);
+                    }
+                    part = wave_sum(part);
+                    if (lane == 0) {
+                        const uint32_t row = p.tile_ptr[t1] & ~kDirty;
+                        y[static_cast<size_t>(row - p.row_base)] = carry[0] + part;
+                    }
+                }
+            } else {
+                cflags |= kChunkTail;
+                if (lane == 0) {
+                    V* tl = static_cast<V*>(p.tail) + static_cast<size_t>(c) * k;
+#pragma unroll
+                    for (int j = 0; j < KMAX; ++j)
+                        if (j < k)
+                            tl[j] = carry[j];
+                }
             }
         }
     }
-    if (!chunk_starts && !carry_is_head)
+    if (!chunk_starts && !carry_is_head && !drop_head)
         cflags |= kChunkHead;
+    if (row_start_seen)
+        cflags |= kChunkRowStart;
     if (lane == 0)
         p.chunk_flags[c] = cflags;
+    if (look) {
+        // empty rows, spread over the chunks
+        for (uint32_t i = p.empty_begin + c * 64 + lane; i < p.empty_end; i += p.nchunks * 64)
+            y[static_cast<size_t>(p.empty_rows[i] - p.row_base)] = V(0);
+    }
     if constexpr (DOT) {
 #pragma unroll
         for (int j = 0; j < KMAX; ++j) {
@@ -473,7 +630,7 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
         if (cf & kChunkHead) {
             // walk back over chunks that lie entirely inside this row
             int64_t kk = static_cast<int64_t>(c) - 1;
-            while (kk >= 0 && !(p.chunk_flags[kk] & (kChunkStarts | kChunkHead)))
+            while (kk >= 0 && !(p.chunk_flags[kk] & kChunkRowStart))
                 --kk;
             const uint32_t from = kk >= 0 ? static_cast<uint32_t>(kk) : 0u;
             const uint32_t row = p.tile_ptr[p.tile_begin + c * p.tpw] & ~kDirty;
@@ -587,6 +744,18 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
     const Plan pl = dot ? plan_by_sigma<V, KMAX, true>(A.sigma, ntiles) : plan_by_sigma<V, KMAX, false>(A.sigma, ntiles);
     tp.tpw = fp.tpw = pl.tpw;
     tp.nchunks = fp.nchunks = pl.nchunks;
+    // Single-vector products without a dot finish rows across chunk
+    // boundaries by look-ahead and write the empty rows themselves; the fix-up
+    // kernel then runs only when some boundary needs it.
+    bool fixup = true;
+    if (KMAX == 1 && !dot && pl.tgrid) {
+        tp.lookahead = 1;
+        fp.empty_end = fp.empty_begin;
+        fixup = !A.binfo || A.binfo->needs_fixup(A.binfo_base + tp.tile_begin, A.binfo_base + tp.tile_end, pl.tpw);
+    } else {
+        tp.lookahead = 0;
+        tp.empty_end = tp.empty_begin;
+    }
     if (pl.tgrid) {
         if (dot)
             launch_tiles_by_sigma<V, KMAX, true>(A.sigma, tp, pl.tgrid, st);
@@ -597,7 +766,7 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
         (void)hipEventRecord(mid, st);
     if (dot)
         hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, true>), dim3(pl.fgrid), dim3(256), 0, st, fp);
-    else
+    else if (fixup)
         hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, false>), dim3(pl.fgrid), dim3(256), 0, st, fp);
     return pl.fgrid;
 }
