@@ -65,7 +65,12 @@ dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out) {
         DIM_HIP_TRY(upload(h.get(), p, f.bit_flag));
         d.bit_flag = p;
     }
-    DIM_HIP_TRY(upload(h.get(), d.tile_flags, f.tile_flags));
+    {
+        // flag words are read 4 bytes at a time: pad to a multiple of 4 plus 4
+        std::vector<uint8_t> flags(f.tile_flags);
+        flags.resize((flags.size() + 3) / 4 * 4 + 4, 0);
+        DIM_HIP_TRY(upload(h.get(), d.tile_flags, flags));
+    }
     h->binfo.assign(f.tile_flags);
     d.binfo = &h->binfo;
     DIM_HIP_TRY(upload(h.get(), d.tile_desc_offset_ptr, f.tile_desc_offset_ptr));

@@ -259,7 +259,10 @@ dim_status convert(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* r
                            bits);
 
     // tile info, dirty segment counts, their scan
-    DIM_HIP_TRY(dev_alloc(h.get(), d.tile_flags, size_t{T} + 1));
+    // flag words are read 4 bytes at a time: pad to a multiple of 4 plus 4
+    const size_t flag_bytes = (size_t{T} + 1 + 3) / 4 * 4 + 4;
+    DIM_HIP_TRY(dev_alloc(h.get(), d.tile_flags, flag_bytes));
+    DIM_HIP_TRY(hipMemsetAsync(d.tile_flags, 0, flag_bytes, s));
     uint32_t* nseg_dirty = nullptr;
     DIM_HIP_TRY(talloc(reinterpret_cast<void**>(&nseg_dirty), (size_t{T} + 1) * 4));
     DIM_HIP_TRY(hipMemsetAsync(nseg_dirty, 0, (size_t{T} + 1) * 4, s));

@@ -147,6 +147,21 @@ __device__ __forceinline__ V ldg_stream(const V* p) {
     return *p;
 }
 
+// Per-tile metadata read at wave-uniform addresses goes through the constant
+// address space, so that the loads are scalar (SMEM) loads that neither take
+// vector registers nor queue behind the vector loads; these arrays are never
+// written while a product runs. Tile flags are read as the aligned 32-bit word
+// that holds them (flag arrays carry at least 4 bytes of padding).
+template<typename T>
+__device__ __forceinline__ T sload(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)(p);
+}
+__device__ __forceinline__ uint32_t sload_flag(const uint8_t* p) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    const uint32_t w = sload(reinterpret_cast<const uint32_t*>(a & ~uintptr_t{3}));
+    return (w >> ((a & 3u) * 8u)) & 0xffu;
+}
+
 // per-chunk flags written by the tile kernel
 constexpr uint8_t kChunkStarts = 1; // the chunk's first element starts a row
 constexpr uint8_t kChunkHead = 2;   // head slot holds the closing part of a row from earlier chunks
@@ -205,7 +220,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         carry[j] = V(0);
         dacc[j] = 0.0;
     }
-    const uint8_t tf0 = p.tile_flags[t0];
+    const uint32_t tf0 = sload_flag(p.tile_flags + t0);
     const bool chunk_starts = tf0 & kStartsRow;
     bool carry_is_head = !chunk_starts;
     uint8_t cflags = chunk_starts ? kChunkStarts : 0;
@@ -221,7 +236,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     const bool look = LOOK && p.lookahead;
     bool drop_head = false;
     if (look && !chunk_starts && t0 > p.tile_begin && (tf0 & kMultiSeg))
-        drop_head = (p.tile_flags[t0 - 1] & (kStartsRow | kMultiSeg)) != 0;
+        drop_head = (sload_flag(p.tile_flags + t0 - 1) & (kStartsRow | kMultiSeg)) != 0;
 
     // Single-vector products keep the lane's whole column in registers so
     // that all its loads are in flight at once. For short tiles they also run
@@ -245,7 +260,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         uint32_t col[NREG];
         W bits;
         uint32_t tile_ptr, tdo_at;
-        uint8_t tf, tf_next;
+        uint32_t tf, tf_next;
     };
     auto fetch = [&](uint32_t t, Stage& s) {
         const size_t base = static_cast<size_t>(t) * (64 * SIGMA) + lane;
@@ -258,10 +273,10 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         }
         s.bits = bits_all[static_cast<size_t>(t) * 64 + lane];
         // tile_flags and tile_desc_offset_ptr hold T + 1 entries
-        s.tile_ptr = p.tile_ptr[t];
-        s.tdo_at = p.tdo_ptr[t];
-        s.tf = p.tile_flags[t];
-        s.tf_next = p.tile_flags[t + 1];
+        s.tile_ptr = sload(p.tile_ptr + t);
+        s.tdo_at = sload(p.tdo_ptr + t);
+        s.tf = sload_flag(p.tile_flags + t);
+        s.tf_next = sload_flag(p.tile_flags + t + 1);
     };
     Stage nx;
     if constexpr (PIPE)
@@ -283,7 +298,8 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             // return in order, so waiting for the gathers then leaves the next
             // tile's loads in flight
             __builtin_amdgcn_sched_barrier(0);
-            fetch(t + 1 < t1 ? t + 1 : t, nx);
+            if (t + 1 < t1)
+                fetch(t + 1, nx);
         } else {
             fetch(t, cu);
             if constexpr (REG) {
@@ -314,6 +330,65 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         }
 
         const int nb = __popc(bits);
+
+        // Fast path for the common tile of short rows: no empty rows, no
+        // padding, and one or two row starts in every lane. The step loop then
+        // needs no branches: the lane's leading partial (red) closes the row
+        // left open by the lane below, a second start closes one row inside
+        // the lane (mid), and the trailing partial continues into the lane
+        // above.
+        if (REG && !dirty && t != p.last_tile && __ballot(nb == 0 || nb > 2) == 0ull) {
+            if constexpr (REG) {
+                const int f1 = __builtin_ctz(bits);
+                const uint32_t rest = bits & (bits - 1u);
+                const int f2 = rest ? __builtin_ctz(rest) : 64;
+                V acc = V(0), red = V(0), mid = V(0);
+#pragma unroll
+                for (int r = 0; r < SIGMA; ++r) {
+                    acc = fma(cu.v[r], xg[r], (r == f1 || r == f2) ? V(0) : acc);
+                    red = (r + 1 == f1) ? acc : red;
+                    mid = (r + 1 == f2) ? acc : mid;
+                }
+                const int incl = wave_incl_scan(nb, lane);
+                const int y_off = incl - nb; // segment of the lane's first start
+                const int nseg = lane_value(incl, 63);
+                const V tot = dpp_val<kDppWaveShr1, 0xf>(acc) + red; // lanes >= 1: segment y_off - 1
+                const size_t yrow0 = static_cast<size_t>(row0 - p.row_base);
+                auto store_row = [&](int seg, V val) {
+                    y[yrow0 + seg] = val;
+                    if constexpr (DOT)
+                        dacc[0] += static_cast<double>(x[row0 + seg]) * static_cast<double>(val);
+                };
+                // segment 0 is lane 0's mid when lane 0 holds two starts,
+                // otherwise lane 1's tot; it continues the carried row unless
+                // the tile starts a row
+                const bool two0 = lane_value(nb, 0) == 2;
+                const bool keep0 = starts_row;
+                if (lane >= 1 && (keep0 || lane != 1 || two0))
+                    store_row(y_off - 1, tot);
+                if (nb == 2 && (keep0 || lane != 0))
+                    store_row(y_off, mid);
+                if (!keep0) {
+                    const V s0 = carry[0] + (two0 ? lane_value(mid, 0) : lane_value(tot, 1));
+                    if (lane == 0) {
+                        if (!carry_is_head)
+                            store_row(0, s0);
+                        else if (!drop_head)
+                            static_cast<V*>(p.head)[c] = s0;
+                    }
+                    carry_is_head = false;
+                }
+                // the tile's last segment starts in lane 63
+                const V tv = lane_value(acc, 63);
+                if (complete) {
+                    if (lane == 63)
+                        store_row(nseg - 1, tv);
+                    carry[0] = V(0);
+                } else {
+                    carry[0] = tv;
+                }
+            }
+        } else {
         const int incl = wave_incl_scan(nb, lane);
         const int y_off = incl - nb; // first segment that starts in this lane
         const int nseg = lane_value(incl, 63);
@@ -529,6 +604,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             process(std::true_type{});
         else
             process(std::false_type{});
+        }
 
         if (t + 1 == t1 && !complete) {
             // the chunk ends inside a row
@@ -553,7 +629,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                     }
                     part = wave_sum(part);
                     if (lane == 0) {
-                        const uint32_t row = p.tile_ptr[t1] & ~kDirty;
+                        const uint32_t row = sload(p.tile_ptr + t1) & ~kDirty;
                         y[static_cast<size_t>(row - p.row_base)] = carry[0] + part;
                     }
                 }
