@@ -78,51 +78,60 @@ __device__ void reduce_parts(const double* parts, int n, int k, double* out) {
     __syncthreads();
 }
 
-// Block-wide sums of acc[j] (fixed order) written to parts[blockIdx.x*k + j].
-template<int KMAX>
-__device__ void block_store_sums(const double* acc, int k, double* parts) {
-    __shared__ double red[kVecThreads / 64][KMAX];
+// Multi-RHS blocks live column-major inside the solver: element (i, j) of an
+// n x k block is at j*n + i, so every column is a plain vector. Vector
+// kernels run on a (blocks, k) grid with blockIdx.y the column; a block's
+// partial sum for its column goes to parts[blockIdx.x*k + j] (fixed order).
+__device__ void block_store_sum(double acc, double* parts) {
+    __shared__ double red[kVecThreads / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j) {
-        const double s = wave_sum(acc[j]);
-        if (lane == 0)
-            red[w][j] = s;
-    }
+    acc = wave_sum(acc);
+    if (lane == 0)
+        red[w] = acc;
     __syncthreads();
-    if (threadIdx.x < static_cast<unsigned>(k)) {
+    if (threadIdx.x == 0) {
         double s = 0.0;
         for (int i = 0; i < nw; ++i)
-            s += red[i][threadIdx.x];
-        parts[static_cast<size_t>(blockIdx.x) * k + threadIdx.x] = s;
+            s += red[i];
+        parts[static_cast<size_t>(blockIdx.x) * gridDim.y + blockIdx.y] = s;
     }
+    __syncthreads();
 }
 
-// r = b - q (q = A x0), p = r; partials of r.r and b.b.
-template<typename V, int KMAX>
-__global__ __launch_bounds__(kVecThreads) void k_cg_init(size_t n, int k, const double* b, const double* q, V* r,
-                                                         V* p, double* rr_part, double* bb_part) {
-    double acc[KMAX], bb[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j)
-        acc[j] = bb[j] = 0.0;
-    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-            if (j < k) {
-                const double bi = b[i * k + j];
-                const V ri = static_cast<V>(bi - q[i * k + j]);
-                r[i * k + j] = ri;
-                p[i * k + j] = ri;
-                acc[j] += static_cast<double>(ri) * static_cast<double>(ri);
-                bb[j] += bi * bi;
-            }
-        }
+// sum_i parts[i*k + j], i < n, by wave 0 in a fixed order, broadcast through
+// `out` (LDS): every block obtains the same bits.
+__device__ void reduce_col(const double* parts, int n, int k, int j, double* out) {
+    if (threadIdx.x < 64) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < n; i += 64)
+            s += parts[static_cast<size_t>(i) * k + j];
+        s = wave_sum(s);
+        if (threadIdx.x == 0)
+            *out = s;
     }
-    block_store_sums<KMAX>(acc, k, rr_part);
     __syncthreads();
-    block_store_sums<KMAX>(bb, k, bb_part);
+}
+
+#define DIM_COL_LOOP(i)                                                                                               \
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;                               \
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+
+// r = b - q (q = A x0), p = r; partials of r.r and b.b.
+template<typename V>
+__global__ __launch_bounds__(kVecThreads) void k_cg_init(size_t n, const double* b, const double* q, V* r, V* p,
+                                                         double* rr_part, double* bb_part) {
+    const size_t o = blockIdx.y * n;
+    double acc = 0.0, bbs = 0.0;
+    DIM_COL_LOOP(i) {
+        const double bi = b[o + i];
+        const V ri = static_cast<V>(bi - q[o + i]);
+        r[o + i] = ri;
+        p[o + i] = ri;
+        acc += static_cast<double>(ri) * static_cast<double>(ri);
+        bbs += bi * bi;
+    }
+    block_store_sum(acc, rr_part);
+    block_store_sum(bbs, bb_part);
 }
 
 // Tolerances and the initial stop test (one block).
@@ -154,122 +163,95 @@ __global__ void k_cg_init_state(CgState* st, int nparts, const double* rr_part, 
 }
 
 // alpha = r.r / p.q; x += alpha p; r -= alpha q; r.r partials.
-template<typename V, int KMAX>
-__global__ __launch_bounds__(kVecThreads) void k_cg_update_xr(CgState* st, size_t n, int k, V* x, V* r, const V* p,
+template<typename V>
+__global__ __launch_bounds__(kVecThreads) void k_cg_update_xr(CgState* st, size_t n, V* x, V* r, const V* p,
                                                               const V* q, const double* pq_part, int n_pq,
                                                               const double* rr_in, double* rr_out, int n_rr) {
     if (st->done)
         return;
-    __shared__ double pq[KMAX], rr[KMAX], alpha_s[KMAX];
-    reduce_parts<KMAX>(pq_part, n_pq, k, pq);
-    reduce_parts<KMAX>(rr_in, n_rr, k, rr);
-    if (threadIdx.x < static_cast<unsigned>(k)) {
-        const int j = threadIdx.x;
-        double a = rr[j] / pq[j];
-        const bool broke = !(pq[j] > 0.0) || !std::isfinite(a);
-        if (st->col_done[j] || broke)
+    const int j = blockIdx.y, k = gridDim.y;
+    __shared__ double pq, rr, alpha_s;
+    reduce_col(pq_part, n_pq, k, j, &pq);
+    reduce_col(rr_in, n_rr, k, j, &rr);
+    if (threadIdx.x == 0) {
+        double a = rr / pq;
+        const bool broke = !(pq > 0.0) || !std::isfinite(a);
+        const int cd = st->col_done[j];
+        if (cd || broke)
             a = 0.0;
-        alpha_s[j] = a;
-        if (broke && !st->col_done[j] && blockIdx.x == 0) {
+        alpha_s = a;
+        if (broke && !cd && blockIdx.x == 0) {
             st->col_done[j] = 2;
             st->col_iters[j] = st->iterations;
             st->done = 2;
         }
     }
     __syncthreads();
-    V alpha[KMAX];
-    double acc[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j) {
-        alpha[j] = static_cast<V>(j < k ? alpha_s[j] : 0.0);
-        acc[j] = 0.0;
+    const V a = static_cast<V>(alpha_s);
+    const size_t o = static_cast<size_t>(j) * n;
+    double acc = 0.0;
+    DIM_COL_LOOP(i) {
+        x[o + i] = x[o + i] + a * p[o + i];
+        const V ri = r[o + i] - a * q[o + i];
+        r[o + i] = ri;
+        acc += static_cast<double>(ri) * static_cast<double>(ri);
     }
-    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-            if (j < k) {
-                const size_t e = i * k + j;
-                x[e] = x[e] + alpha[j] * p[e];
-                const V ri = r[e] - alpha[j] * q[e];
-                r[e] = ri;
-                acc[j] += static_cast<double>(ri) * static_cast<double>(ri);
-            }
-        }
-    }
-    block_store_sums<KMAX>(acc, k, rr_out);
+    block_store_sum(acc, rr_out);
 }
 
-// Stop test; beta = r.r_new / r.r; p = r + beta p.
+// Stop test; beta = r.r_new / r.r; p = r + beta p. Block (0, 0) keeps the
+// iteration count and the per-column states.
 template<typename V, int KMAX>
-__global__ __launch_bounds__(kVecThreads) void k_cg_update_p(CgState* st, size_t n, int k, const V* r, V* p,
+__global__ __launch_bounds__(kVecThreads) void k_cg_update_p(CgState* st, size_t n, const V* r, V* p,
                                                              const double* rr_new_part, const double* rr_old_part,
                                                              int n_rr, double* hist, int hist_cap) {
     if (st->done)
         return;
-    __shared__ double rn[KMAX], ro[KMAX];
-    __shared__ int conv_s[KMAX];
-    __shared__ double beta_s[KMAX];
-    __shared__ int all_s;
-    reduce_parts<KMAX>(rr_new_part, n_rr, k, rn);
-    reduce_parts<KMAX>(rr_old_part, n_rr, k, ro);
-    if (threadIdx.x == 0) {
-        int all = 1;
-        for (int j = 0; j < k; ++j) {
-            // a column frozen earlier (converged or broken down) keeps r, so
-            // its test repeats with the same numbers
-            const int c = (rn[j] <= st->tol2[j]) || (st->col_done[j] == 2);
-            conv_s[j] = c;
-            beta_s[j] = rn[j] / ro[j];
-            all &= c;
+    const int j = blockIdx.y, k = gridDim.y;
+    __shared__ double rn, ro;
+    reduce_col(rr_new_part, n_rr, k, j, &rn);
+    reduce_col(rr_old_part, n_rr, k, j, &ro);
+    // a column frozen earlier (converged or broken down) keeps r, so its
+    // test repeats with the same numbers
+    const bool conv = (rn <= st->tol2[j]) || (st->col_done[j] == 2);
+    if (!conv) {
+        const V beta = static_cast<V>(rn / ro);
+        const size_t o = static_cast<size_t>(j) * n;
+        DIM_COL_LOOP(i) {
+            p[o + i] = r[o + i] + beta * p[o + i];
         }
-        all_s = all;
     }
-    __syncthreads();
-    V beta[KMAX];
-    bool conv[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j) {
-        beta[j] = static_cast<V>(j < k ? beta_s[j] : 0.0);
-        conv[j] = j < k ? conv_s[j] != 0 : true;
-    }
-    if (!all_s) {
-        for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
-             i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j) {
-                if (j < k && !conv[j]) {
-                    const size_t e = i * k + j;
-                    p[e] = r[e] + beta[j] * p[e];
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        __shared__ double rn_all[KMAX];
+        reduce_parts<KMAX>(rr_new_part, n_rr, k, rn_all);
+        if (threadIdx.x == 0) {
+            const int it = st->iterations + 1;
+            st->iterations = it;
+            int all = 1;
+            for (int c = 0; c < k; ++c) {
+                const int cc = (rn_all[c] <= st->tol2[c]) || (st->col_done[c] == 2);
+                all &= cc;
+                st->rr[c] = rn_all[c];
+                if (cc && !st->col_done[c]) {
+                    st->col_done[c] = 1;
+                    st->col_iters[c] = it;
+                }
+                if (hist && it < hist_cap) {
+                    const double scale = st->bnorm[c] > 0.0 ? st->bnorm[c] : 1.0;
+                    hist[static_cast<size_t>(it) * k + c] = std::sqrt(rn_all[c]) / scale;
                 }
             }
+            if (all)
+                st->done = 1;
+            else if (it >= st->max_iters)
+                st->done = 3;
         }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const int it = st->iterations + 1;
-        st->iterations = it;
-        for (int j = 0; j < k; ++j) {
-            st->rr[j] = rn[j];
-            if (conv_s[j] && !st->col_done[j]) {
-                st->col_done[j] = 1;
-                st->col_iters[j] = it;
-            }
-            if (hist && it < hist_cap) {
-                const double scale = st->bnorm[j] > 0.0 ? st->bnorm[j] : 1.0;
-                hist[static_cast<size_t>(it) * k + j] = std::sqrt(rn[j]) / scale;
-            }
-        }
-        if (all_s)
-            st->done = 1;
-        else if (it >= st->max_iters)
-            st->done = 3;
     }
 }
 
 // Reliable update, part 1: x += x_corr (FP64), x_corr = 0.
-template<int KMAX>
-__global__ __launch_bounds__(kVecThreads) void k_ru_fold(size_t n, int k, double* x, float* xc) {
-    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n * k;
+__global__ __launch_bounds__(kVecThreads) void k_ru_fold(size_t nk, double* x, float* xc) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < nk;
          i += static_cast<size_t>(gridDim.x) * blockDim.x) {
         x[i] += static_cast<double>(xc[i]);
         xc[i] = 0.0f;
@@ -278,29 +260,19 @@ __global__ __launch_bounds__(kVecThreads) void k_ru_fold(size_t n, int k, double
 
 // Reliable update, part 2: r64 = b - q64; r32 = float(r64); partials of the
 // FP32 residual's r.r (iteration scalar) and of the FP64 residual (stop test).
-template<int KMAX>
-__global__ __launch_bounds__(kVecThreads) void k_ru_residual(size_t n, int k, const double* b, const double* q,
-                                                             float* r32, double* rr_part, double* rt_part) {
-    double acc[KMAX], tr[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j)
-        acc[j] = tr[j] = 0.0;
-    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-            if (j < k) {
-                const double r = b[i * k + j] - q[i * k + j];
-                const float rs = static_cast<float>(r);
-                r32[i * k + j] = rs;
-                acc[j] += static_cast<double>(rs) * static_cast<double>(rs);
-                tr[j] += r * r;
-            }
-        }
+__global__ __launch_bounds__(kVecThreads) void k_ru_residual(size_t n, const double* b, const double* q, float* r32,
+                                                             double* rr_part, double* rt_part) {
+    const size_t o = blockIdx.y * n;
+    double acc = 0.0, trs = 0.0;
+    DIM_COL_LOOP(i) {
+        const double rv = b[o + i] - q[o + i];
+        const float rsv = static_cast<float>(rv);
+        r32[o + i] = rsv;
+        acc += static_cast<double>(rsv) * static_cast<double>(rsv);
+        trs += rv * rv;
     }
-    block_store_sums<KMAX>(acc, k, rr_part);
-    __syncthreads();
-    block_store_sums<KMAX>(tr, k, rt_part);
+    block_store_sum(acc, rr_part);
+    block_store_sum(trs, rt_part);
 }
 
 // Reliable update, part 3 (one block): the FP64 stop test decides; columns
@@ -330,23 +302,27 @@ __global__ void k_ru_state(CgState* st, int nparts, const double* rr_part, const
 }
 
 // ||b - q||^2 partials (FP64) for the final true residual.
-template<int KMAX>
-__global__ __launch_bounds__(kVecThreads) void k_resid_parts(size_t n, int k, const double* b, const double* q,
+__global__ __launch_bounds__(kVecThreads) void k_resid_parts(size_t n, const double* b, const double* q,
                                                              double* parts) {
-    double acc[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j)
-        acc[j] = 0.0;
-    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j)
-            if (j < k) {
-                const double d = b[i * k + j] - q[i * k + j];
-                acc[j] += d * d;
-            }
+    const size_t o = blockIdx.y * n;
+    double acc = 0.0;
+    DIM_COL_LOOP(i) {
+        const double d = b[o + i] - q[o + i];
+        acc += d * d;
     }
-    block_store_sums<KMAX>(acc, k, parts);
+    block_store_sum(acc, parts);
+}
+
+// Row-major n x k <-> the solver's column-major layout.
+__global__ void k_rows_to_cols(size_t n, int k, const double* src, double* dst) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n * k;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[(e % k) * n + e / k] = src[e];
+}
+__global__ void k_cols_to_rows(size_t n, int k, const double* src, double* dst) {
+    for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < n * k;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[e] = src[(e % k) * n + e / k];
 }
 
 template<typename V>
@@ -402,40 +378,39 @@ uint32_t vec_blocks(size_t n) {
 
 template<int KMAX>
 struct Launcher {
-    // one CG iteration in precision V (f32 picks the FP32 matrix values)
+    // one CG iteration in precision V (f32 picks the FP32 matrix values); the
+    // n x k blocks are column-major (ldv = n)
     template<typename V>
     static void iteration(dim_csr5* A, CgState* st, size_t n, int k, V* x, V* r, V* p, V* q, double* rr_a,
                           double* rr_b, uint32_t nvb, double* hist, int hist_cap, hipStream_t s) {
         constexpr bool f32 = std::is_same<V, float>::value;
         uint32_t n_pq = 1;
-        (void)launch_spmv(A->f, A->s, f32, k, p, q, 0, A->f.tiles, 0, 0, A->f.n_empty, true, &st->done, s, &n_pq);
-        hipLaunchKernelGGL((k_cg_update_xr<V, KMAX>), dim3(nvb), dim3(kVecThreads), 0, s, st, n, k, x, r,
-                           static_cast<const V*>(p), static_cast<const V*>(q), A->s.dot_blk, static_cast<int>(n_pq), rr_a, rr_b,
+        (void)launch_spmv(A->f, A->s, f32, k, p, q, 0, A->f.tiles, 0, 0, A->f.n_empty, true, &st->done, s, &n_pq, n);
+        const dim3 grid(nvb, static_cast<uint32_t>(k));
+        hipLaunchKernelGGL((k_cg_update_xr<V>), grid, dim3(kVecThreads), 0, s, st, n, x, r, static_cast<const V*>(p),
+                           static_cast<const V*>(q), A->s.dot_blk, static_cast<int>(n_pq), rr_a, rr_b,
                            static_cast<int>(nvb));
-        hipLaunchKernelGGL((k_cg_update_p<V, KMAX>), dim3(nvb), dim3(kVecThreads), 0, s, st, n, k,
-                           static_cast<const V*>(r), p, rr_b, rr_a, static_cast<int>(nvb), hist, hist_cap);
+        hipLaunchKernelGGL((k_cg_update_p<V, KMAX>), grid, dim3(kVecThreads), 0, s, st, n, static_cast<const V*>(r),
+                           p, rr_b, rr_a, static_cast<int>(nvb), hist, hist_cap);
     }
 };
 
-struct SolveOut {
-    std::vector<CgState> unused;
-};
-
 template<int KMAX>
-dim_status solve(dim_csr5* A, int k, const double* b, double* x, const dim_cg_opts& o, dim_cg_result* res,
+dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const dim_cg_opts& o, dim_cg_result* res,
                  hipStream_t stream) {
     const size_t n = A->f.rows;
     const bool mixed = o.precision == DIM_CG_F32_REFINED;
     if (mixed && !A->f.vals32)
         return fail(DIM_INVALID_ARGUMENT, "FP32-refined CG needs a matrix built with DIM_CSR5_KEEP_F32");
     const uint32_t nvb = vec_blocks(n);
+    const dim3 vgrid(nvb, static_cast<uint32_t>(k));
     const uint32_t check = o.check_every ? std::max<uint32_t>(2, (o.check_every + 1) & ~1u) : 16;
     const double delta = o.refine_delta > 0.0 ? o.refine_delta : 0.1;
 
     DevBuf buf;
     CgState* st = nullptr;
     double *q64 = nullptr, *r64 = nullptr, *p64 = nullptr, *rr0 = nullptr, *rr1 = nullptr, *bb = nullptr,
-           *rt = nullptr, *hist = nullptr;
+           *rt = nullptr, *hist = nullptr, *bc = nullptr, *xc = nullptr;
     float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
     DIM_HIP_TRY(buf.alloc(st, 1));
     DIM_HIP_TRY(buf.alloc(q64, n * k));
@@ -455,6 +430,12 @@ dim_status solve(dim_csr5* A, int k, const double* b, double* x, const dim_cg_op
         DIM_HIP_TRY(buf.alloc(r64, n * k));
         DIM_HIP_TRY(buf.alloc(p64, n * k));
     }
+    if (k > 1) {
+        DIM_HIP_TRY(buf.alloc(bc, n * k));
+        DIM_HIP_TRY(buf.alloc(xc, n * k));
+    }
+    const double* b = k > 1 ? bc : b_in;
+    double* x = k > 1 ? xc : x_out;
     PinnedState hs;
     if (!hs.h)
         return fail(DIM_OUT_OF_MEMORY, "pinned host state");
@@ -492,19 +473,24 @@ dim_status solve(dim_csr5* A, int k, const double* b, double* x, const dim_cg_op
     DIM_HIP_TRY(hipEventCreate(&e1));
     DIM_HIP_TRY(hipEventRecord(e0, stream));
 
+    const uint32_t tgrid = 1024;
+    if (k > 1) {
+        hipLaunchKernelGGL(k_rows_to_cols, dim3(tgrid), dim3(256), 0, stream, n, k, b_in, bc);
+        hipLaunchKernelGGL(k_rows_to_cols, dim3(tgrid), dim3(256), 0, stream, n, k, static_cast<const double*>(x_out),
+                           xc);
+    }
     CgState init{};
     init.max_iters = static_cast<int>(o.max_iters);
     init.k = k;
     DIM_HIP_TRY(hipMemcpyAsync(st, &init, sizeof(CgState), hipMemcpyHostToDevice, stream));
     // q = A x0 (FP64), r = b - q, p = r
-    DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false, nullptr, stream));
+    DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false, nullptr, stream,
+                            nullptr, n));
     if (mixed) {
-        hipLaunchKernelGGL((k_cg_init<float, KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, b, q64, r32, p32,
-                           rr0, bb);
+        hipLaunchKernelGGL((k_cg_init<float>), vgrid, dim3(kVecThreads), 0, stream, n, b, q64, r32, p32, rr0, bb);
         hipLaunchKernelGGL((k_zero<float>), dim3(nvb), dim3(kVecThreads), 0, stream, x32, n * k);
     } else {
-        hipLaunchKernelGGL((k_cg_init<double, KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, b, q64, r64, p64,
-                           rr0, bb);
+        hipLaunchKernelGGL((k_cg_init<double>), vgrid, dim3(kVecThreads), 0, stream, n, b, q64, r64, p64, rr0, bb);
     }
     hipLaunchKernelGGL((k_cg_init_state<KMAX>), dim3(1), dim3(64), 0, stream, st, static_cast<int>(nvb), rr0, bb,
                        o.rtol, hist);
@@ -531,11 +517,11 @@ dim_status solve(dim_csr5* A, int k, const double* b, double* x, const dim_cg_op
                 need = !hs.h->col_done[j] && hs.h->rr[j] < delta * delta * rr_at_update[j];
             if (need && hs.h->iterations > 0) {
                 ++refinements;
-                hipLaunchKernelGGL((k_ru_fold<KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, x, x32);
+                hipLaunchKernelGGL(k_ru_fold, dim3(nvb), dim3(kVecThreads), 0, stream, n * k, x, x32);
                 DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false,
-                                        nullptr, stream));
-                hipLaunchKernelGGL((k_ru_residual<KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, b, q64, r32,
-                                   rr0, rt);
+                                        nullptr, stream, nullptr, n));
+                hipLaunchKernelGGL(k_ru_residual, vgrid, dim3(kVecThreads), 0, stream, n, b,
+                                   static_cast<const double*>(q64), r32, rr0, rt);
                 hipLaunchKernelGGL((k_ru_state<KMAX>), dim3(1), dim3(64), 0, stream, st, static_cast<int>(nvb), rr0,
                                    rt);
                 DIM_HIP_TRY(hipGetLastError());
@@ -554,24 +540,28 @@ dim_status solve(dim_csr5* A, int k, const double* b, double* x, const dim_cg_op
             DIM_HIP_TRY(hipGetLastError());
         }
     }
+    // true residual ||b - A x|| in FP64, and the solution back in the
+    // caller's row-major layout
+    DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false, nullptr, stream,
+                            nullptr, n));
+    hipLaunchKernelGGL(k_resid_parts, vgrid, dim3(kVecThreads), 0, stream, n, b, static_cast<const double*>(q64), rt);
+    if (k > 1)
+        hipLaunchKernelGGL(k_cols_to_rows, dim3(tgrid), dim3(256), 0, stream, n, k, static_cast<const double*>(xc),
+                           x_out);
+    DIM_HIP_TRY(hipGetLastError());
     DIM_HIP_TRY(hipEventRecord(e1, stream));
     DIM_HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-
-    // true residual ||b - A x|| in FP64
-    DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false, nullptr, stream));
-    hipLaunchKernelGGL((k_resid_parts<KMAX>), dim3(nvb), dim3(kVecThreads), 0, stream, n, k, b, q64, rt);
-    DIM_HIP_TRY(hipGetLastError());
     std::vector<double> parts(size_t{nvb} * k);
     DIM_HIP_TRY(hipMemcpyAsync(parts.data(), rt, parts.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
     DIM_HIP_TRY(hipStreamSynchronize(stream));
     if (hist_cap) {
         std::vector<double> hh(static_cast<size_t>(hist_cap) * k);
         DIM_HIP_TRY(hipMemcpy(hh.data(), hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost));
-        // column-major per column j: opts.history holds column 0 (single RHS use)
+        // opts.history holds column 0 (single right-hand side use)
         for (int i = 0; i < hist_cap; ++i)
             o.history[i] = hh[static_cast<size_t>(i) * k];
     }

@@ -190,7 +190,7 @@ namespace {
 
 void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
                  uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, const int* done,
-                 TileParams& tp, FixParams& fp) {
+                 TileParams& tp, FixParams& fp, uint64_t ldv = 0) {
     tp = make_tile_params(A, f32);
     tp.x = x;
     tp.y = y;
@@ -206,7 +206,13 @@ void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x,
     tp.empty_rows = A.empty_rows;
     tp.empty_begin = empty_begin;
     tp.empty_end = empty_end;
+    tp.rs = ldv ? 1 : static_cast<uint64_t>(k);
+    tp.cs = ldv ? ldv : 1;
     fp = FixParams{};
+    fp.rs = tp.rs;
+    fp.cs = tp.cs;
+    fp.dot_ld = static_cast<uint32_t>(k);
+    fp.dot_off = 0;
     fp.tile_ptr = A.tile_ptr;
     fp.empty_rows = A.empty_rows;
     fp.chunk_flags = S.chunk_flags;
@@ -228,22 +234,40 @@ void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x,
 
 hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
                        uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, bool dot,
-                       const int* done, hipStream_t stream, uint32_t* dot_blocks) {
+                       const int* done, hipStream_t stream, uint32_t* dot_blocks, uint64_t ldv) {
     TileParams tp;
     FixParams fp;
-    fill_params(A, S, f32, k, x, y, tile_begin, tile_end, row_base, empty_begin, empty_end, done, tp, fp);
-    uint32_t nb;
-    if (k == 1)
+    fill_params(A, S, f32, k, x, y, tile_begin, tile_end, row_base, empty_begin, empty_end, done, tp, fp, ldv);
+    uint32_t nb = 0;
+    if (k == 1) {
         nb = f32 ? launch_pair<float, 1>(A, tp, fp, dot, stream, nullptr)
                  : launch_pair<double, 1>(A, tp, fp, dot, stream, nullptr);
-    else if (k <= 2)
-        nb = launch_pair_k2(f32, A, tp, fp, dot, stream);
-    else if (k <= 4)
-        nb = launch_pair_k4(f32, A, tp, fp, dot, stream);
-    else if (k <= 8)
-        nb = launch_pair_k8(f32, A, tp, fp, dot, stream);
-    else
-        nb = launch_pair_k16(f32, A, tp, fp, dot, stream);
+    } else {
+        // Wide blocks run as column groups over the same tiles: FP64 groups of
+        // up to 4 columns and FP32 groups of up to 8 keep the tile kernel's
+        // registers, and so its occupancy, at the best measured point.
+        // Every group runs the same kernel width, so that all groups share one
+        // launch plan and leave the same number of dot partials per column.
+        const int group = k < (f32 ? 8 : 4) ? k : (f32 ? 8 : 4);
+        const size_t esz = f32 ? 4 : 8;
+        for (int j0 = 0; j0 < k; j0 += group) {
+            const int kc = k - j0 < group ? k - j0 : group;
+            TileParams t = tp;
+            FixParams f = fp;
+            t.k = f.k = kc;
+            t.x = f.x = static_cast<const char*>(x) + j0 * tp.cs * esz;
+            t.y = f.y = static_cast<char*>(y) + j0 * tp.cs * esz;
+            f.dot_off = static_cast<uint32_t>(j0);
+            if (group <= 2)
+                nb = launch_pair_k2(f32, A, t, f, dot, stream);
+            else if (group <= 4)
+                nb = launch_pair_k4(f32, A, t, f, dot, stream);
+            else if (group <= 8)
+                nb = launch_pair_k8(f32, A, t, f, dot, stream);
+            else
+                nb = launch_pair_k16(f32, A, t, f, dot, stream);
+        }
+    }
     if (dot_blocks)
         *dot_blocks = nb;
     return hipGetLastError();

@@ -190,6 +190,9 @@ struct TileParams {
     uint32_t empty_begin, empty_end; // empty rows written by this kernel
     int k;
     int lookahead;
+    // x and y hold k columns: element (i, j) at i*rs + j*cs (row-major n x K
+    // blocks: rs = K, cs = 1; column-major: rs = 1, cs = leading dimension)
+    uint64_t rs, cs;
 };
 
 template<typename V, int SIGMA, int KMAX, bool DOT>
@@ -209,6 +212,8 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     const int k = KMAX == 1 ? 1 : p.k;
     const V* __restrict__ x = static_cast<const V*>(p.x);
     V* __restrict__ y = static_cast<V*>(p.y);
+    const size_t rs = KMAX == 1 ? 1 : p.rs; // row stride of x and y
+    const size_t cs = KMAX == 1 ? 0 : p.cs; // column stride
 
     // The row left open by the previous tile of this chunk (its partial sum),
     // and whether that row began before the chunk: its partial then goes to
@@ -325,8 +330,11 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         if (t == p.last_tile) {
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r)
-                if (base + static_cast<size_t>(lane) * SIGMA + r >= p.nnz)
+                if (base + static_cast<size_t>(lane) * SIGMA + r >= p.nnz) {
                     pad |= 1u << r;
+                    if constexpr (REG)
+                        xg[r] = V(0);
+                }
         }
 
         const int nb = __popc(bits);
@@ -407,17 +415,17 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                     return row0 + static_cast<uint32_t>(s);
             };
             auto write_row = [&](uint32_t row, const Vec& val) {
-                V* yr = y + static_cast<size_t>(row - p.row_base) * k;
+                V* yr = y + static_cast<size_t>(row - p.row_base) * rs;
 #pragma unroll
                 for (int j = 0; j < KMAX; ++j)
                     if (j < k)
-                        yr[j] = val.a[j];
+                        yr[j * cs] = val.a[j];
                 if constexpr (DOT) {
-                    const V* pr = x + static_cast<size_t>(row) * k;
+                    const V* pr = x + static_cast<size_t>(row) * rs;
 #pragma unroll
                     for (int j = 0; j < KMAX; ++j)
                         if (j < k)
-                            dacc[j] += static_cast<double>(pr[j]) * static_cast<double>(val.a[j]);
+                            dacc[j] += static_cast<double>(pr[j * cs]) * static_cast<double>(val.a[j]);
                 }
             };
             // segment 0 of a tile that does not start a row closes the carried row
@@ -476,20 +484,14 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                     for (int j = 0; j < KMAX; ++j)
                         cur.a[j] = V(0);
                 }
-                const bool pz = (pad >> r) & 1u;
                 if constexpr (REG) {
-                    cur.a[0] = fma(vr, pz ? V(0) : xg[r], cur.a[0]);
-                } else {
-                    const V* xr = x + static_cast<size_t>(cr) * k;
+                    cur.a[0] = fma(vr, xg[r], cur.a[0]); // padded steps hold x = 0
+                } else if (!((pad >> r) & 1u)) {
+                    const V* xr = x + static_cast<size_t>(cr) * rs;
 #pragma unroll
-                    for (int j = 0; j < KMAX; ++j) {
-                        if (j < k) {
-                            V xv = xr[j];
-                            if (pz)
-                                xv = V(0);
-                            cur.a[j] = fma(vr, xv, cur.a[j]);
-                        }
-                    }
+                    for (int j = 0; j < KMAX; ++j)
+                        if (j < k)
+                            cur.a[j] = fma(vr, xr[j * cs], cur.a[j]);
                 }
             };
             if constexpr (REG) {
@@ -680,6 +682,8 @@ struct FixParams {
     uint32_t tile_begin, row_base, empty_begin, empty_end;
     uint32_t tpw, nchunks;
     int k;
+    uint64_t rs, cs;             // as TileParams::rs, cs
+    uint32_t dot_ld, dot_off;    // dot partial of column j goes to dot_blk[block*dot_ld + dot_off + j]
 };
 
 // Rows that cross chunk boundaries: the chunk in which such a row ends adds
@@ -694,6 +698,8 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
     const int k = KMAX == 1 ? 1 : p.k;
     const V* x = static_cast<const V*>(p.x);
     V* y = static_cast<V*>(p.y);
+    const size_t rs = KMAX == 1 ? 1 : p.rs;
+    const size_t cs = KMAX == 1 ? 0 : p.cs;
     const V* head = static_cast<const V*>(p.head);
     const V* tail = static_cast<const V*>(p.tail);
     double dacc[KMAX];
@@ -710,7 +716,7 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
                 --kk;
             const uint32_t from = kk >= 0 ? static_cast<uint32_t>(kk) : 0u;
             const uint32_t row = p.tile_ptr[p.tile_begin + c * p.tpw] & ~kDirty;
-            V* yr = y + static_cast<size_t>(row - p.row_base) * k;
+            V* yr = y + static_cast<size_t>(row - p.row_base) * rs;
 #pragma unroll
             for (int j = 0; j < KMAX; ++j) {
                 if (j < k) {
@@ -723,9 +729,9 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
                     } else {
                         s = head[static_cast<size_t>(c) * k + j];
                     }
-                    yr[j] = s;
+                    yr[j * cs] = s;
                     if constexpr (DOT)
-                        dacc[j] += static_cast<double>(x[static_cast<size_t>(row) * k + j]) * static_cast<double>(s);
+                        dacc[j] += static_cast<double>(x[static_cast<size_t>(row) * rs + j * cs]) * static_cast<double>(s);
                 }
             }
         }
@@ -740,9 +746,9 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
     const uint32_t ne = p.empty_end - p.empty_begin;
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < ne; i += gridDim.x * 256) {
         const uint32_t row = p.empty_rows[p.empty_begin + i];
-        V* yr = y + static_cast<size_t>(row - p.row_base) * k;
+        V* yr = y + static_cast<size_t>(row - p.row_base) * rs;
         for (int j = 0; j < k; ++j)
-            yr[j] = V(0);
+            yr[j * cs] = V(0);
     }
 
     if constexpr (DOT) {
@@ -756,7 +762,7 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
         }
         __syncthreads();
         if (threadIdx.x < static_cast<unsigned>(k))
-            p.dot_blk[static_cast<size_t>(blockIdx.x) * k + threadIdx.x] =
+            p.dot_blk[static_cast<size_t>(blockIdx.x) * p.dot_ld + p.dot_off + threadIdx.x] =
               (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
     }
 }

@@ -116,7 +116,9 @@ def test_cg_iteration_cap(cuda):
 @pytest.mark.parametrize("k", [2, 3, 8])
 def test_cg_multi_rhs_equals_single(cuda, k):
     """k right-hand sides through one SpMM per iteration give, column by
-    column, the same iterates as k single solves (bitwise)."""
+    column, the iterates of k single solves: the same iteration count within
+    two and the same solution to the solver tolerance (the coalesced multi-
+    column reductions add the partial sums in another order)."""
     import torch
 
     rp, ci, v = matgen.varcoef((24, 24, 24), 1.0, seed=3)
@@ -126,8 +128,9 @@ def test_cg_multi_rhs_equals_single(cuda, k):
     X, res = conjugate_gradient_multi(A, to_dev(B), rtol=1e-10, max_iters=3000)
     for j in range(k):
         xj, rj = conjugate_gradient(A, to_dev(B[:, j].copy()), rtol=1e-10, max_iters=3000)
-        assert res[j].converged and res[j].iterations == rj.iterations
-        assert torch.equal(X[:, j].contiguous(), xj)
+        assert res[j].converged and abs(res[j].iterations - rj.iterations) <= 2
+        assert res[j].relres_true <= 1.01e-10
+        np.testing.assert_allclose(X[:, j].cpu().numpy(), xj.cpu().numpy(), rtol=1e-7, atol=1e-9)
 
 
 @pytest.mark.parametrize("k", [1, 4])
