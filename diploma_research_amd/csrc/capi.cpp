@@ -162,7 +162,11 @@ dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint
                           const uint32_t* col_idx, const double* vals, bool keep_f32, dim_csr5** out) {
     const std::vector<uint32_t> candidates = sigma_candidates(rows, nnz);
     dim_csr5* best = nullptr;
-    float best_ms = 0;
+    struct Tried {
+        dim_csr5* h;
+        float ms;
+    };
+    std::vector<Tried> tried;
     double* x = nullptr;
     double* y = nullptr;
     DIM_HIP_TRY(hipMalloc(&x, std::max<size_t>(cols, 1) * sizeof(double)));
@@ -195,31 +199,52 @@ dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint
             break;
         for (int i = 0; i < 3; ++i)
             (void)launch_spmv(h->f, h->s, false, 1, x, y, 0, h->f.tiles, 0, 0, h->f.n_empty, false, nullptr, 0);
+        // best of three rounds of ten products: the rounds absorb clock and
+        // queue noise that a single round shows
         const int reps = 10;
-        (void)hipEventRecord(e0, 0);
-        for (int i = 0; i < reps; ++i)
-            (void)launch_spmv(h->f, h->s, false, 1, x, y, 0, h->f.tiles, 0, 0, h->f.n_empty, false, nullptr, 0);
-        (void)hipEventRecord(e1, 0);
-        hipError_t e = hipEventSynchronize(e1);
+        float ms = 0;
+        hipError_t e = hipSuccess;
+        for (int round = 0; round < 3 && e == hipSuccess; ++round) {
+            (void)hipEventRecord(e0, 0);
+            for (int i = 0; i < reps; ++i)
+                (void)launch_spmv(h->f, h->s, false, 1, x, y, 0, h->f.tiles, 0, 0, h->f.n_empty, false, nullptr,
+                                  0);
+            (void)hipEventRecord(e1, 0);
+            e = hipEventSynchronize(e1);
+            float r = 0;
+            (void)hipEventElapsedTime(&r, e0, e1);
+            if (round == 0 || r < ms)
+                ms = r;
+        }
         if (e != hipSuccess) {
             free_handle(h);
             st = hip_fail(e, "autotune timing");
             break;
         }
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (!best || ms < best_ms) {
-            free_handle(best);
-            best = h;
-            best_ms = ms;
-        } else {
-            free_handle(h);
-        }
+        tried.push_back({h, ms});
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(x);
     (void)hipFree(y);
+    if (st == DIM_SUCCESS && !tried.empty()) {
+        // Candidates within 3% of the fastest are ties (timing noise): among
+        // them take the sigma nearest the mean row length, so that repeated
+        // conversions of one matrix pick the same format.
+        float fastest = tried[0].ms;
+        for (const auto& c : tried)
+            fastest = std::min(fastest, c.ms);
+        const double mean_len = rows ? static_cast<double>(nnz) / rows : 0.0;
+        for (const auto& c : tried) {
+            if (c.ms > 1.03f * fastest)
+                continue;
+            if (!best || std::abs(c.h->f.sigma - mean_len) < std::abs(best->f.sigma - mean_len))
+                best = c.h;
+        }
+    }
+    for (const auto& c : tried)
+        if (c.h != best)
+            free_handle(c.h);
     if (st != DIM_SUCCESS) {
         free_handle(best);
         return st;
@@ -370,14 +395,17 @@ dim_status dim_csr5_spmv_timed(const dim_csr5* A, dim_dtype dtype, const void* x
     for (auto& e : ev)
         DIM_HIP_TRY(hipEventCreate(&e));
     auto* self = const_cast<dim_csr5*>(A);
+    bool fixup = true;
     hipError_t e = launch_spmv_events(self->f, self->s, dtype == DIM_F32, x, y, ev[0], ev[1], ev[2],
-                                      static_cast<hipStream_t>(stream));
+                                      static_cast<hipStream_t>(stream), &fixup);
     if (e == hipSuccess)
         e = hipEventSynchronize(ev[2]);
     if (e == hipSuccess)
         e = hipEventElapsedTime(&ms[0], ev[0], ev[1]);
     if (e == hipSuccess)
         e = hipEventElapsedTime(&ms[1], ev[1], ev[2]);
+    if (!fixup)
+        ms[1] = 0.0f;
     for (auto& ee : ev)
         (void)hipEventDestroy(ee);
     if (e != hipSuccess)

@@ -81,7 +81,7 @@ hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const vo
 // Same as launch_spmv over the whole matrix with events recorded before the
 // tile kernel, between the two kernels and after the fix-up kernel.
 hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const void* x, void* y, hipEvent_t e0,
-                              hipEvent_t e1, hipEvent_t e2, hipStream_t stream);
+                              hipEvent_t e1, hipEvent_t e2, hipStream_t stream, bool* fixup_launched = nullptr);
 
 // y = A^T x (x has rows entries, y has cols entries); y is zeroed first.
 hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, void* y, hipStream_t stream,

@@ -274,17 +274,17 @@ hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const vo
 }
 
 hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const void* x, void* y, hipEvent_t e0,
-                              hipEvent_t e1, hipEvent_t e2, hipStream_t stream) {
+                              hipEvent_t e1, hipEvent_t e2, hipStream_t stream, bool* fixup_launched) {
     TileParams tp;
     FixParams fp;
     fill_params(A, S, f32, 1, x, y, 0, A.tiles, 0, 0, A.n_empty, nullptr, tp, fp);
     hipError_t e = hipEventRecord(e0, stream);
     if (e != hipSuccess)
         return e;
-    if (f32)
-        launch_pair<float, 1>(A, tp, fp, false, stream, e1);
-    else
-        launch_pair<double, 1>(A, tp, fp, false, stream, e1);
+    const uint32_t nfix =
+      f32 ? launch_pair<float, 1>(A, tp, fp, false, stream, e1) : launch_pair<double, 1>(A, tp, fp, false, stream, e1);
+    if (fixup_launched)
+        *fixup_launched = nfix != 0;
     e = hipEventRecord(e2, stream);
     if (e != hipSuccess)
         return e;

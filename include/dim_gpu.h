@@ -99,7 +99,8 @@ dim_status dim_csr5_spmv(const dim_csr5* A, dim_dtype dtype, const void* x, uint
 
 /* Measurement variant of dim_csr5_spmv: records HIP events around the tile
  * kernel and the carry/fix-up kernel on `stream`, waits for them and returns
- * both durations in milliseconds (ms[0] tiles, ms[1] fix-up). No reference
+ * both durations in milliseconds (ms[0] tiles, ms[1] fix-up; ms[1] is 0 when
+ * the product needs no fix-up kernel and none was launched). No reference
  * counterpart (the reference times whole calls with a stopwatch,
  * include/dim/bench/stopwatch.h:11-22). */
 dim_status dim_csr5_spmv_timed(const dim_csr5* A, dim_dtype dtype, const void* x, uint64_t x_len, void* y,
