@@ -1,7 +1,7 @@
-"""Build recipe for libdim_gpu.so (gfx950) and the dim CLI.
+"""Build recipe for libdim_gpu.so (gfx950), libdim_h5.so and the dim CLI.
 
-    python -m diploma_research_amd.build          # library + CLI
-    python -m diploma_research_amd.build --lib    # library only
+    python -m diploma_research_amd.build          # libraries + CLI
+    python -m diploma_research_amd.build --lib    # libdim_gpu.so only
 
 Every HIP/C++ source is compiled by hipcc for gfx950 in parallel into
 _build/, then linked in-tree as diploma_research_amd/libdim_gpu.so.
@@ -11,7 +11,6 @@ from __future__ import annotations
 import argparse
 import concurrent.futures as cf
 import os
-import shutil
 import subprocess
 import sys
 
@@ -43,25 +42,26 @@ def _run(cmd):
     return p.returncode, out
 
 
-def _compile(src):
+def _compile(src, force=False):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     path = os.path.join(CSRC, src)
     deps = [path] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
     deps.append(os.path.join(ROOT, "include", "dim_gpu.h"))
-    if os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
+    if not force and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
         return src, 0, "(up to date)"
     lang = ["-x", "hip"] if src.endswith(".hip") else []
     rc, out = _run([HIPCC, *HIP_FLAGS, *lang, "-c", path, "-o", obj])
     return src, rc, out
 
 
-def build_lib(jobs=None, verbose=False):
+def build_lib(jobs=None, verbose=False, force=False):
+    """Compiles every source (force) or the out-of-date ones, then links."""
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     objs = []
     failed = []
     with cf.ThreadPoolExecutor(jobs) as ex:
-        for src, rc, out in ex.map(_compile, SOURCES):
+        for src, rc, out in ex.map(lambda f: _compile(f, force), SOURCES):
             objs.append(os.path.join(BUILD, os.path.basename(src) + ".o"))
             if rc != 0:
                 failed.append((src, out))
@@ -79,18 +79,35 @@ def build_lib(jobs=None, verbose=False):
     return LIB
 
 
-def build_cli():
-    src = os.path.join(PKG, "cli", "dim_cli.cpp")
-    if not os.path.exists(src):
-        return None
+H5LIB = os.path.join(PKG, "libdim_h5.so")
+CXX = os.environ.get("CXX", "g++")
+
+
+def build_h5():
+    """libdim_h5.so (include/dim_h5.h): HDF5 matrix/vector files, host only."""
     inc = os.path.join(HDF5_ROOT, "include")
     libdir = os.path.join(HDF5_ROOT, "lib")
     if not os.path.exists(os.path.join(inc, "hdf5.h")):
-        print(f"HDF5 headers not found under {HDF5_ROOT}; skipping the dim CLI", file=sys.stderr)
+        print(f"HDF5 headers not found under {HDF5_ROOT}; skipping libdim_h5 and the dim CLI", file=sys.stderr)
+        return None
+    tmp = H5LIB + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc, "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "h5io.cpp"), "-o", tmp, "-L", libdir, "-lhdf5", f"-Wl,-rpath,{libdir}"]
+    rc, out = _run(cmd)
+    if rc != 0:
+        raise RuntimeError("libdim_h5 build failed:\n" + out)
+    os.replace(tmp, H5LIB)
+    return H5LIB
+
+
+def build_cli():
+    """The dim command-line tool (links libdim_gpu and libdim_h5 in-tree)."""
+    src = os.path.join(PKG, "cli", "dim_cli.cpp")
+    if not os.path.exists(src) or build_h5() is None:
         return None
     tmp = CLI + ".tmp"
-    cmd = [HIPCC, "-O2", "-std=c++17", "-I", inc, "-I", os.path.join(ROOT, "include"), src, "-o", tmp,
-           "-L", libdir, "-lhdf5", f"-Wl,-rpath,{libdir}", "-L", PKG, "-ldim_gpu", "-Wl,-rpath,$ORIGIN"]
+    cmd = [HIPCC, "-O2", "-std=c++20", "-Wall", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"), src,
+           "-o", tmp, "-L", PKG, "-ldim_gpu", "-ldim_h5", "-Wl,-rpath,$ORIGIN"]
     rc, out = _run(cmd)
     if rc != 0:
         raise RuntimeError("dim CLI build failed:\n" + out)
@@ -101,13 +118,11 @@ def build_cli():
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true", help="library only")
-    ap.add_argument("--clean", action="store_true")
+    ap.add_argument("--clean", action="store_true", help="recompile every source")
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("-v", action="store_true")
     a = ap.parse_args(argv)
-    if a.clean and os.path.isdir(BUILD):
-        shutil.rmtree(BUILD)
-    print(build_lib(a.j, a.v))
+    print(build_lib(a.j, a.v, force=a.clean))
     if not a.lib:
         print(build_cli())
 

@@ -47,7 +47,8 @@ typedef struct dim_csr5 dim_csr5;
 
 /* Construction options. */
 typedef struct dim_csr5_opts {
-    uint32_t sigma; /* tile height: 8, 16, 24 or 32; 0 = pick by the built-in autotuner */
+    uint32_t sigma; /* tile height: 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28 or 32; 0 = pick by the
+                       built-in autotuner (times the candidates near the mean row length) */
     uint32_t flags; /* DIM_CSR5_* below */
 } dim_csr5_opts;
 #define DIM_CSR5_KEEP_F32 1u /* also keep FP32 values (for DIM_F32 products and mixed-precision CG) */

@@ -263,3 +263,95 @@ def ref_csr_from_coo(rows, cols, ri, ci, v, symmetric=False):
     n = ref().dimref_csr_from_coo(rows, cols, nnz, int(symmetric), np.ascontiguousarray(ri, np.uint32),
                                   np.ascontiguousarray(ci, np.uint32), np.ascontiguousarray(v, np.float64), rp, oc, ov)
     return rp, oc[:n], ov[:n]
+
+
+# ---------------------------------------------------------------------------
+# The reference's own HDF5 I/O (oracle/_ref/libdimref_h5.so, built from
+# libs/dim/io/h5.cpp and friends against the HDF5 C library). Used by the tests
+# to write files the way the reference's store-matrix does and to load files
+# the way its benchmarks do.
+
+_ref_h5 = None
+
+
+def ref_h5_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libdimref_h5.so"))
+
+
+def _rh5():
+    global _ref_h5
+    if _ref_h5 is None:
+        L = C.CDLL(os.path.join(_HERE, "_ref", "libdimref_h5.so"))
+        for name in ("dimref_h5_store_csr5", "dimref_h5_store_matlab"):
+            f = getattr(L, name)
+            f.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, _u32p, _u32p, _f64p]
+            f.restype = C.c_int
+        L.dimref_h5_load_csr5.argtypes = [C.c_char_p, C.c_char_p]
+        L.dimref_h5_load_csr5.restype = C.c_void_p
+        L.dimref_h5_csr5_free.argtypes = [C.c_void_p]
+        L.dimref_h5_csr5_sizes.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+        L.dimref_h5_csr5_copy.argtypes = [C.c_void_p, _u32p, _u32p, _u32p, _u32p, _u32p, _f64p, _u32p]
+        L.dimref_h5_read_matlab.argtypes = [C.c_char_p, C.c_char_p]
+        L.dimref_h5_read_matlab.restype = C.c_void_p
+        L.dimref_h5_csr_free.argtypes = [C.c_void_p]
+        L.dimref_h5_csr_sizes.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+        L.dimref_h5_csr_copy.argtypes = [C.c_void_p, _u32p, _u32p, _f64p]
+        _ref_h5 = L
+    return _ref_h5
+
+
+def _csr_args(rows, cols, rp, ci, v):
+    rp = np.ascontiguousarray(rp, dtype=np.uint32)
+    ci = np.ascontiguousarray(ci, dtype=np.uint32)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return rows, cols, len(v), rp, ci, v
+
+
+def ref_h5_store_csr5(path, group, rows, cols, rp, ci, v, append=False):
+    """h5::store(csr5::from_csr(csr)) as store_matrix -f csr5 does."""
+    if _rh5().dimref_h5_store_csr5(os.fspath(path).encode(), group.encode(), int(append),
+                                   *_csr_args(rows, cols, rp, ci, v)):
+        raise RuntimeError("reference h5::store failed")
+
+
+def ref_h5_store_matlab(path, group, rows, cols, rp, ci, v, append=False):
+    if _rh5().dimref_h5_store_matlab(os.fspath(path).encode(), group.encode(), int(append),
+                                     *_csr_args(rows, cols, rp, ci, v)):
+        raise RuntimeError("reference write_matlab_compatible failed")
+
+
+def ref_h5_load_csr5(path, group="A") -> dict:
+    """h5::load_csr5: the arrays exactly as stored (tile-transposed values,
+    packed tile_desc words)."""
+    L = _rh5()
+    h = L.dimref_h5_load_csr5(os.fspath(path).encode(), group.encode())
+    if not h:
+        raise RuntimeError("reference load_csr5 failed")
+    try:
+        sz = np.zeros(5, np.uint64)
+        L.dimref_h5_csr5_sizes(h, sz)
+        rows, cols, nnz, T, ntdo = (int(s) for s in sz)
+        out = dict(rows=rows, cols=cols, row_ptr=np.zeros(rows + 1, np.uint32), tile_ptr=np.zeros(T + 1, np.uint32),
+                   tile_desc=np.zeros(4 * T, np.uint32), tile_desc_offset_ptr=np.zeros(T + 1, np.uint32),
+                   tile_desc_offset=np.zeros(ntdo, np.uint32), vals=np.zeros(nnz), col_idx=np.zeros(nnz, np.uint32))
+        L.dimref_h5_csr5_copy(h, out["row_ptr"], out["tile_ptr"], out["tile_desc"], out["tile_desc_offset_ptr"],
+                              out["tile_desc_offset"], out["vals"], out["col_idx"])
+        return out
+    finally:
+        L.dimref_h5_csr5_free(h)
+
+
+def ref_h5_read_matlab(path, group="A"):
+    L = _rh5()
+    h = L.dimref_h5_read_matlab(os.fspath(path).encode(), group.encode())
+    if not h:
+        raise RuntimeError("reference read_matlab_compatible failed")
+    try:
+        sz = np.zeros(3, np.uint64)
+        L.dimref_h5_csr_sizes(h, sz)
+        rows, cols, nnz = (int(s) for s in sz)
+        rp, ci, v = np.zeros(rows + 1, np.uint32), np.zeros(nnz, np.uint32), np.zeros(nnz)
+        L.dimref_h5_csr_copy(h, rp, ci, v)
+        return rows, cols, rp, ci, v
+    finally:
+        L.dimref_h5_csr_free(h)
