@@ -411,8 +411,14 @@ int spmv(int argc, char** argv) {
     } else {
         HIPCHECK(hipMemcpy(x, xh.data(), xh.size() * 8, hipMemcpyHostToDevice));
     }
+    auto calibrator = A.create_calibrator();
     auto run = [&] {
-        dim::gpu::check(dim_csr5_spmv(A.handle(), f32 ? DIM_F32 : DIM_F64, x, m.cols, y, m.rows, nullptr));
+        if (f32)
+            A.spmv({.x = {static_cast<const float*>(x), m.cols}, .y = {static_cast<float*>(y), m.rows},
+                    .calibrator = calibrator});
+        else
+            A.spmv({.x = {static_cast<const double*>(x), m.cols}, .y = {static_cast<double*>(y), m.rows},
+                    .calibrator = calibrator});
     };
     for (int i = 0; i < 5; ++i) // warm-up (main.cpp:60-62)
         run();
