@@ -303,15 +303,32 @@ class Csr5:
         return first, y
 
     def spmm(self, X, Y=None, stream=None):
-        """Y = A*X for X of shape (cols, k), k in 1..16, row-major."""
+        """Y = A*X for X of shape (cols, k), k in 1..16.
+
+        Column-major X (stride (1, ld), e.g. the transpose of a contiguous
+        (k, cols) tensor) runs the column-major product, the faster layout;
+        a new Y then has the same layout. Anything else is made row-major."""
         torch = _torch()
         if X.dim() != 2 or X.shape[0] != self.dimensions.cols:
             raise ValueError("X must have shape (cols, k)")
-        k = X.shape[1]
-        X = X.contiguous()
+        rows, k = self.dimensions.rows, X.shape[1]
+        colmajor = k > 1 and X.stride(0) == 1 and X.stride(1) >= max(X.shape[0], 1)
+        if not colmajor:
+            X = X.contiguous()
         if Y is None:
-            Y = torch.empty((self.dimensions.rows, k), dtype=X.dtype, device=X.device)
-        N.check(N.lib().dim_csr5_spmm(self._h, _dtype_code(X), k, _ptr(X), _ptr(Y), _stream_ptr(stream)))
+            Y = (torch.empty((k, rows), dtype=X.dtype, device=X.device).t() if colmajor else
+                 torch.empty((rows, k), dtype=X.dtype, device=X.device))
+        if Y.shape != (rows, k) or Y.dtype != X.dtype:
+            raise ValueError("Y must have shape (rows, k) and the dtype of X")
+        if colmajor:
+            if Y.stride(0) != 1 or Y.stride(1) < max(rows, 1):
+                raise ValueError("Y must be column-major like X")
+            N.check(N.lib().dim_csr5_spmm_colmajor(self._h, _dtype_code(X), k, _ptr(X), X.stride(1), _ptr(Y),
+                                                   Y.stride(1), _stream_ptr(stream)))
+        else:
+            if not Y.is_contiguous():
+                raise ValueError("Y must be row-major (contiguous) like X")
+            N.check(N.lib().dim_csr5_spmm(self._h, _dtype_code(X), k, _ptr(X), _ptr(Y), _stream_ptr(stream)))
         return Y
 
     def spmv_transpose(self, x, y=None, stream=None):

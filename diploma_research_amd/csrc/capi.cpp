@@ -480,6 +480,24 @@ dim_status dim_csr5_spmm(const dim_csr5* A, dim_dtype dtype, uint32_t k, const v
     return DIM_SUCCESS;
 }
 
+dim_status dim_csr5_spmm_colmajor(const dim_csr5* A, dim_dtype dtype, uint32_t k, const void* X, uint64_t ldx,
+                                  void* Y, uint64_t ldy, void* stream) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (dim_status st = check_dtype(A, dtype))
+        return st;
+    if (k < 1 || k > static_cast<uint32_t>(kMaxK))
+        return fail(DIM_INVALID_ARGUMENT, "k must be in 1..16");
+    if ((A->f.cols && !X) || (A->f.rows && !Y))
+        return fail(DIM_INVALID_ARGUMENT, "null matrix argument");
+    if (ldx < std::max<uint64_t>(A->f.cols, 1) || ldy < std::max<uint64_t>(A->f.rows, 1))
+        return fail(DIM_INVALID_ARGUMENT, "leading dimension smaller than the column length");
+    auto* self = const_cast<dim_csr5*>(A);
+    DIM_HIP_TRY(launch_spmv(self->f, self->s, dtype == DIM_F32, static_cast<int>(k), X, Y, 0, A->f.tiles, 0, 0,
+                            A->f.n_empty, false, nullptr, static_cast<hipStream_t>(stream), nullptr, ldx, ldy));
+    return DIM_SUCCESS;
+}
+
 dim_status dim_csr5_spmv_transpose(const dim_csr5* A, dim_dtype dtype, const void* x, void* y, void* stream) {
     if (dim_status st = check_handle(A))
         return st;

@@ -71,12 +71,13 @@ struct Scratch {
 // fix-up block; *dot_blocks (if given) receives that count.
 // `done` (may be null): a device flag; when *done != 0 the kernels return at
 // once (used by solver launch sequences).
-// ldv: 0 for row-major n x k blocks (element (i, j) at i*k + j); otherwise the
-// blocks are column-major with element (i, j) at i + j*ldv.
+// ldx: 0 for row-major n x k blocks (element (i, j) at i*k + j); otherwise the
+// blocks are column-major with element (i, j) of x at i + j*ldx and of y at
+// i + j*ldy (ldy = 0: same as ldx).
 hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y,
                        uint32_t tile_begin, uint32_t tile_end, uint32_t row_base, uint32_t empty_begin,
                        uint32_t empty_end, bool dot, const int* done, hipStream_t stream,
-                       uint32_t* dot_blocks = nullptr, uint64_t ldv = 0);
+                       uint32_t* dot_blocks = nullptr, uint64_t ldx = 0, uint64_t ldy = 0);
 
 // Same as launch_spmv over the whole matrix with events recorded before the
 // tile kernel, between the two kernels and after the fix-up kernel.

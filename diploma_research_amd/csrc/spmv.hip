@@ -188,9 +188,13 @@ int detail::device_cus() {
 
 namespace {
 
+// widest multi-RHS block decoded in one pass over the tiles
+constexpr int kColumnGroupF64 = 16;
+constexpr int kColumnGroupF32 = 16;
+
 void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
                  uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, const int* done,
-                 TileParams& tp, FixParams& fp, uint64_t ldv = 0) {
+                 TileParams& tp, FixParams& fp, uint64_t ldx = 0, uint64_t ldy = 0) {
     tp = make_tile_params(A, f32);
     tp.x = x;
     tp.y = y;
@@ -206,11 +210,13 @@ void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x,
     tp.empty_rows = A.empty_rows;
     tp.empty_begin = empty_begin;
     tp.empty_end = empty_end;
-    tp.rs = ldv ? 1 : static_cast<uint64_t>(k);
-    tp.cs = ldv ? ldv : 1;
+    tp.rs = ldx ? 1 : static_cast<uint64_t>(k);
+    tp.xcs = ldx ? ldx : 1;
+    tp.ycs = ldx ? (ldy ? ldy : ldx) : 1;
     fp = FixParams{};
     fp.rs = tp.rs;
-    fp.cs = tp.cs;
+    fp.xcs = tp.xcs;
+    fp.ycs = tp.ycs;
     fp.dot_ld = static_cast<uint32_t>(k);
     fp.dot_off = 0;
     fp.tile_ptr = A.tile_ptr;
@@ -234,29 +240,29 @@ void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x,
 
 hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
                        uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, bool dot,
-                       const int* done, hipStream_t stream, uint32_t* dot_blocks, uint64_t ldv) {
+                       const int* done, hipStream_t stream, uint32_t* dot_blocks, uint64_t ldx, uint64_t ldy) {
     TileParams tp;
     FixParams fp;
-    fill_params(A, S, f32, k, x, y, tile_begin, tile_end, row_base, empty_begin, empty_end, done, tp, fp, ldv);
+    fill_params(A, S, f32, k, x, y, tile_begin, tile_end, row_base, empty_begin, empty_end, done, tp, fp, ldx, ldy);
     uint32_t nb = 0;
     if (k == 1) {
         nb = f32 ? launch_pair<float, 1>(A, tp, fp, dot, stream, nullptr)
                  : launch_pair<double, 1>(A, tp, fp, dot, stream, nullptr);
     } else {
-        // Wide blocks run as column groups over the same tiles: FP64 groups of
-        // up to 4 columns and FP32 groups of up to 8 keep the tile kernel's
-        // registers, and so its occupancy, at the best measured point.
-        // Every group runs the same kernel width, so that all groups share one
-        // launch plan and leave the same number of dot partials per column.
-        const int group = k < (f32 ? 8 : 4) ? k : (f32 ? 8 : 4);
+        // Wide blocks may run as column groups over the same tiles (each group
+        // reads the matrix again). Every group runs the same kernel width, so
+        // that all groups share one launch plan and leave the same number of
+        // dot partials per column.
+        const int gmax = f32 ? kColumnGroupF32 : kColumnGroupF64;
+        const int group = k < gmax ? k : gmax;
         const size_t esz = f32 ? 4 : 8;
         for (int j0 = 0; j0 < k; j0 += group) {
             const int kc = k - j0 < group ? k - j0 : group;
             TileParams t = tp;
             FixParams f = fp;
             t.k = f.k = kc;
-            t.x = f.x = static_cast<const char*>(x) + j0 * tp.cs * esz;
-            t.y = f.y = static_cast<char*>(y) + j0 * tp.cs * esz;
+            t.x = f.x = static_cast<const char*>(x) + j0 * tp.xcs * esz;
+            t.y = f.y = static_cast<char*>(y) + j0 * tp.ycs * esz;
             f.dot_off = static_cast<uint32_t>(j0);
             if (group <= 2)
                 nb = launch_pair_k2(f32, A, t, f, dot, stream);

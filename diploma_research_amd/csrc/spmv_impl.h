@@ -196,10 +196,20 @@ struct TileParams {
     uint32_t empty_begin, empty_end; // empty rows written by this kernel
     int k;
     int lookahead;
-    // x and y hold k columns: element (i, j) at i*rs + j*cs (row-major n x K
-    // blocks: rs = K, cs = 1; column-major: rs = 1, cs = leading dimension)
-    uint64_t rs, cs;
+    // x and y hold k columns: element (i, j) of x at i*rs + j*xcs, of y at
+    // i*rs + j*ycs (row-major n x K blocks: rs = K, xcs = ycs = 1;
+    // column-major: rs = 1, xcs and ycs the leading dimensions)
+    uint64_t rs, xcs, ycs;
 };
+
+// Multi-RHS products: right-hand sides whose gathers are in flight together
+// in the fast path, and the tallest tile whose matrix column a lane keeps in
+// registers.
+template<typename V>
+constexpr int smm_group() {
+    return sizeof(V) == 8 ? 2 : 4;
+}
+constexpr int kSmmRegSigma = 16;
 
 template<typename V, int SIGMA, int KMAX, bool DOT>
 __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
@@ -218,8 +228,9 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     const int k = KMAX == 1 ? 1 : p.k;
     const V* __restrict__ x = static_cast<const V*>(p.x);
     V* __restrict__ y = static_cast<V*>(p.y);
-    const size_t rs = KMAX == 1 ? 1 : p.rs; // row stride of x and y
-    const size_t cs = KMAX == 1 ? 0 : p.cs; // column stride
+    const size_t rs = KMAX == 1 ? 1 : p.rs;   // row stride of x and y
+    const size_t xcs = KMAX == 1 ? 0 : p.xcs; // column strides
+    const size_t ycs = KMAX == 1 ? 0 : p.ycs;
 
     // The row left open by the previous tile of this chunk (its partial sum),
     // and whether that row began before the chunk: its partial then goes to
@@ -249,23 +260,24 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     if (look && !chunk_starts && t0 > p.tile_begin && (tf0 & kMultiSeg))
         drop_head = (sload_flag(p.tile_flags + t0 - 1) & (kStartsRow | kMultiSeg)) != 0;
 
-    // Single-vector products keep the lane's whole column in registers so
-    // that all its loads are in flight at once. For short tiles they also run
+    // A lane keeps its whole column of the tile in registers so that all its
+    // loads are in flight at once (multi-RHS products up to sigma 16; taller
+    // tiles stream the column step by step). Single-vector products also
+    // gather their sigma x entries up front (XREG) and, for short tiles, run
     // a two-stage pipeline over the chunk: the sigma values, column indices,
     // bit flags and tile metadata of tile t+1 are in flight while the x
     // gathers and the scans of tile t run, so the stream of matrix bytes does
     // not stop while a wavefront works on a tile (taller tiles would lose more
-    // to the lower occupancy than they gain). Multi-RHS products stream the
-    // column step by step (their k-wide x rows dominate).
-    constexpr bool REG = KMAX == 1;
-    constexpr bool PIPE = REG && SIGMA <= 8;
+    // to the lower occupancy than they gain). Multi-RHS products gather per
+    // right-hand side inside the tile, after the decode.
+    constexpr bool XREG = KMAX == 1;
+    constexpr bool REG = XREG || SIGMA <= kSmmRegSigma;
+    constexpr bool PIPE = XREG && SIGMA <= 8;
     constexpr int NREG = REG ? SIGMA : 1;
+    constexpr int NX = XREG ? SIGMA : 1;
     const V* __restrict__ vals_all = static_cast<const V*>(p.vals);
     const W* __restrict__ bits_all = static_cast<const W*>(p.bit_flag);
 
-    struct Vec {
-        V a[KMAX];
-    };
     struct Stage {
         V v[NREG];
         uint32_t col[NREG];
@@ -273,16 +285,22 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         uint32_t tile_ptr, tdo_at;
         uint32_t tf, tf_next;
     };
+    // Tile loads add a 32-bit lane offset to a wave-uniform tile base, so that
+    // they take the base from scalar registers (the readfirstlane keeps the
+    // compiler from turning the bases into per-lane 64-bit pointers).
+    const uint32_t ulane = static_cast<uint32_t>(lane);
     auto fetch = [&](uint32_t t, Stage& s) {
-        const size_t base = static_cast<size_t>(t) * (64 * SIGMA) + lane;
+        const size_t base = static_cast<size_t>(__builtin_amdgcn_readfirstlane(t)) * (64 * SIGMA);
         if constexpr (REG) {
+            const V* vt = vals_all + base;
+            const uint32_t* ct = p.col_idx + base;
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r) {
-                s.v[r] = ldg_stream<!DOT>(vals_all + base + r * 64);
-                s.col[r] = ldg_stream<!DOT>(p.col_idx + base + r * 64);
+                s.v[r] = ldg_stream<!DOT>(vt + r * 64 + ulane);
+                s.col[r] = ldg_stream<!DOT>(ct + r * 64 + ulane);
             }
         }
-        s.bits = bits_all[static_cast<size_t>(t) * 64 + lane];
+        s.bits = (bits_all + base / SIGMA)[ulane];
         // tile_flags and tile_desc_offset_ptr hold T + 1 entries
         s.tile_ptr = sload(p.tile_ptr + t);
         s.tdo_at = sload(p.tdo_ptr + t);
@@ -299,7 +317,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         const uint32_t* cols = p.col_idx + base + lane;
 
         Stage cu;
-        V xg[NREG];
+        V xg[NX];
         if constexpr (PIPE) {
             cu = nx;
 #pragma unroll
@@ -313,7 +331,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 fetch(t + 1, nx);
         } else {
             fetch(t, cu);
-            if constexpr (REG) {
+            if constexpr (XREG) {
 #pragma unroll
                 for (int r = 0; r < SIGMA; ++r)
                     xg[r] = x[cu.col[r]];
@@ -334,11 +352,12 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         // padded elements of the last tile contribute exactly zero
         uint32_t pad = 0;
         if (t == p.last_tile) {
+            const uint32_t left = static_cast<uint32_t>(p.nnz - base); // elements of the tile that exist
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r)
-                if (base + static_cast<size_t>(lane) * SIGMA + r >= p.nnz) {
+                if (ulane * SIGMA + r >= left) {
                     pad |= 1u << r;
-                    if constexpr (REG)
+                    if constexpr (XREG)
                         xg[r] = V(0);
                 }
         }
@@ -350,57 +369,99 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         // needs no branches: the lane's leading partial (red) closes the row
         // left open by the lane below, a second start closes one row inside
         // the lane (mid), and the trailing partial continues into the lane
-        // above.
+        // above. Multi-RHS products decode the tile once and run the step loop
+        // once per right-hand side, G sides at a time with their gathers in
+        // flight together.
         if (REG && !dirty && t != p.last_tile && __ballot(nb == 0 || nb > 2) == 0ull) {
             if constexpr (REG) {
                 const int f1 = __builtin_ctz(bits);
                 const uint32_t rest = bits & (bits - 1u);
                 const int f2 = rest ? __builtin_ctz(rest) : 64;
-                V acc = V(0), red = V(0), mid = V(0);
-#pragma unroll
-                for (int r = 0; r < SIGMA; ++r) {
-                    acc = fma(cu.v[r], xg[r], (r == f1 || r == f2) ? V(0) : acc);
-                    red = (r + 1 == f1) ? acc : red;
-                    mid = (r + 1 == f2) ? acc : mid;
-                }
                 const int incl = wave_incl_scan(nb, lane);
                 const int y_off = incl - nb; // segment of the lane's first start
                 const int nseg = lane_value(incl, 63);
-                const V tot = dpp_val<kDppWaveShr1, 0xf>(acc) + red; // lanes >= 1: segment y_off - 1
                 const size_t yrow0 = static_cast<size_t>(row0 - p.row_base);
-                auto store_row = [&](int seg, V val) {
-                    y[yrow0 + seg] = val;
-                    if constexpr (DOT)
-                        dacc[0] += static_cast<double>(x[row0 + seg]) * static_cast<double>(val);
-                };
                 // segment 0 is lane 0's mid when lane 0 holds two starts,
                 // otherwise lane 1's tot; it continues the carried row unless
                 // the tile starts a row
                 const bool two0 = lane_value(nb, 0) == 2;
                 const bool keep0 = starts_row;
-                if (lane >= 1 && (keep0 || lane != 1 || two0))
-                    store_row(y_off - 1, tot);
-                if (nb == 2 && (keep0 || lane != 0))
-                    store_row(y_off, mid);
-                if (!keep0) {
-                    const V s0 = carry[0] + (two0 ? lane_value(mid, 0) : lane_value(tot, 1));
-                    if (lane == 0) {
-                        if (!carry_is_head)
-                            store_row(0, s0);
-                        else if (!drop_head)
-                            static_cast<V*>(p.head)[c] = s0;
+                // The rows a lane may write: segment y_off - 1 (tot), y_off
+                // (mid; in lane 0 segment 0; in lane 63 with one start the
+                // tile's last segment) and y_off + 1 (the tile's last segment
+                // in lane 63 with two starts). The fused dot reads x at them.
+                const size_t ra = row0 + static_cast<uint32_t>(y_off > 0 ? y_off - 1 : 0);
+                const size_t rb = row0 + static_cast<uint32_t>(y_off);
+                const size_t rc = row0 + static_cast<uint32_t>(y_off + 1 < nseg ? y_off + 1 : y_off);
+                constexpr int G = KMAX < smm_group<V>() ? KMAX : smm_group<V>();
+#pragma unroll
+                for (int g = 0; g < KMAX; g += G) {
+                    if (g < k) {
+                        // sides past k re-read side k - 1 and store nothing
+                        V xv[G][SIGMA];
+                        V px[G][3];
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            const V* xj = x + static_cast<size_t>(g + i < k ? g + i : k - 1) * xcs;
+#pragma unroll
+                            for (int r = 0; r < SIGMA; ++r) {
+                                if constexpr (XREG)
+                                    xv[i][r] = xg[r];
+                                else
+                                    xv[i][r] = xj[static_cast<size_t>(cu.col[r]) * rs];
+                            }
+                            if constexpr (DOT) {
+                                px[i][0] = xj[ra * rs];
+                                px[i][1] = xj[rb * rs];
+                                px[i][2] = xj[rc * rs];
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            const int j = g + i;
+                            if (j < k) {
+                                V acc = V(0), red = V(0), mid = V(0);
+#pragma unroll
+                                for (int r = 0; r < SIGMA; ++r) {
+                                    acc = fma(cu.v[r], xv[i][r], (r == f1 || r == f2) ? V(0) : acc);
+                                    red = (r + 1 == f1) ? acc : red;
+                                    mid = (r + 1 == f2) ? acc : mid;
+                                }
+                                const V tot = dpp_val<kDppWaveShr1, 0xf>(acc) + red; // lanes >= 1: segment y_off - 1
+                                V* yj = y + static_cast<size_t>(j) * ycs;
+                                auto store_row = [&](int seg, V val, V xr) {
+                                    yj[(yrow0 + seg) * rs] = val;
+                                    if constexpr (DOT)
+                                        dacc[j] += static_cast<double>(xr) * static_cast<double>(val);
+                                };
+                                if (lane >= 1 && (keep0 || lane != 1 || two0))
+                                    store_row(y_off - 1, tot, px[i][0]);
+                                if (nb == 2 && (keep0 || lane != 0))
+                                    store_row(y_off, mid, px[i][1]);
+                                if (!keep0) {
+                                    const V s0 = carry[j] + (two0 ? lane_value(mid, 0) : lane_value(tot, 1));
+                                    if (lane == 0) {
+                                        if (!carry_is_head)
+                                            store_row(0, s0, px[i][1]);
+                                        else if (!drop_head)
+                                            static_cast<V*>(p.head)[static_cast<size_t>(c) * k + j] = s0;
+                                    }
+                                }
+                                // the tile's last segment starts in lane 63
+                                const V tv = lane_value(acc, 63);
+                                if (complete) {
+                                    if (lane == 63)
+                                        store_row(nseg - 1, tv, nb == 2 ? px[i][2] : px[i][1]);
+                                    carry[j] = V(0);
+                                } else {
+                                    carry[j] = tv;
+                                }
+                            }
+                        }
                     }
+                }
+                if (!keep0)
                     carry_is_head = false;
-                }
-                // the tile's last segment starts in lane 63
-                const V tv = lane_value(acc, 63);
-                if (complete) {
-                    if (lane == 63)
-                        store_row(nseg - 1, tv);
-                    carry[0] = V(0);
-                } else {
-                    carry[0] = tv;
-                }
             }
         } else {
         const int incl = wave_incl_scan(nb, lane);
@@ -411,8 +472,14 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         // The rest of the tile is instantiated twice: tiles without empty rows
         // map segment s to row row0 + s with no memory access, so no load (and
         // no wait for loads) sits between the gathers and the stores.
+        // Multi-RHS tiles run it once per group of GG right-hand sides, so that
+        // only GG partial sums of each kind are live in a lane.
         auto process = [&](auto dirty_tag) {
             constexpr bool DIRTY = decltype(dirty_tag)::value;
+            constexpr int GG = KMAX < smm_group<V>() ? KMAX : smm_group<V>();
+            struct Vec {
+                V a[GG];
+            };
             const uint32_t* tdo = DIRTY ? p.tdo + __builtin_amdgcn_readfirstlane(cu.tdo_at) : nullptr;
             auto row_of = [&](int s) -> uint32_t {
                 if constexpr (DIRTY)
@@ -420,193 +487,201 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 else
                     return row0 + static_cast<uint32_t>(s);
             };
-            auto write_row = [&](uint32_t row, const Vec& val) {
-                V* yr = y + static_cast<size_t>(row - p.row_base) * rs;
+            const bool cih = carry_is_head;
 #pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    if (j < k)
-                        yr[j * cs] = val.a[j];
-                if constexpr (DOT) {
-                    const V* pr = x + static_cast<size_t>(row) * rs;
+            for (int g = 0; g < KMAX; g += GG) {
+                if (g >= k)
+                    break; // wave-uniform
+                const int kg = k - g; // sides i < kg of this group exist
+                auto write_row = [&](uint32_t row, const Vec& val) {
+                    V* yr = y + static_cast<size_t>(row - p.row_base) * rs + static_cast<size_t>(g) * ycs;
 #pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        if (j < k)
-                            dacc[j] += static_cast<double>(pr[j * cs]) * static_cast<double>(val.a[j]);
-                }
-            };
-            // segment 0 of a tile that does not start a row closes the carried row
-            auto close_carried = [&](const Vec& val) {
-                Vec tot;
+                    for (int i = 0; i < GG; ++i)
+                        if (i < kg)
+                            yr[i * ycs] = val.a[i];
+                    if constexpr (DOT) {
+                        const V* pr = x + static_cast<size_t>(row) * rs + static_cast<size_t>(g) * xcs;
 #pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    tot.a[j] = carry[j] + val.a[j];
-                if (carry_is_head) {
-                    if (drop_head)
-                        return; // the previous chunk writes this row
-                    V* h = static_cast<V*>(p.head) + static_cast<size_t>(c) * k;
+                        for (int i = 0; i < GG; ++i)
+                            if (i < kg)
+                                dacc[g + i] += static_cast<double>(pr[i * xcs]) * static_cast<double>(val.a[i]);
+                    }
+                };
+                // segment 0 of a tile that does not start a row closes the carried row
+                auto close_carried = [&](const Vec& val) {
+                    Vec tot;
 #pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        if (j < k)
-                            h[j] = tot.a[j];
-                } else {
-                    write_row(row0, tot);
-                }
-            };
-            // Segment 0 of a tile that does not start a row continues the
-            // carried row. The lane that closes it only keeps its partial;
-            // one lane adds it to the carry after the scan, so that the head
-            // slot and y are written from wave-uniform code only.
-            bool own0 = false;
-            Vec seg0;
+                    for (int i = 0; i < GG; ++i)
+                        tot.a[i] = carry[g + i] + val.a[i];
+                    if (cih) {
+                        if (drop_head)
+                            return; // the previous chunk writes this row
+                        V* h = static_cast<V*>(p.head) + static_cast<size_t>(c) * k + g;
 #pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                seg0.a[j] = V(0);
-            auto emit = [&](int s, const Vec& val) {
-                if (s == 0 && !starts_row) {
-                    own0 = true;
-                    seg0 = val;
-                } else {
-                    write_row(row_of(s), val);
-                }
-            };
+                        for (int i = 0; i < GG; ++i)
+                            if (i < kg)
+                                h[i] = tot.a[i];
+                    } else {
+                        write_row(row0, tot);
+                    }
+                };
+                // Segment 0 of a tile that does not start a row continues the
+                // carried row. The lane that closes it only keeps its partial;
+                // one lane adds it to the carry after the scan, so that the head
+                // slot and y are written from wave-uniform code only.
+                bool own0 = false;
+                Vec seg0;
+#pragma unroll
+                for (int i = 0; i < GG; ++i)
+                    seg0.a[i] = V(0);
+                auto emit = [&](int s, const Vec& val) {
+                    if (s == 0 && !starts_row) {
+                        own0 = true;
+                        seg0 = val;
+                    } else {
+                        write_row(row_of(s), val);
+                    }
+                };
 
-            Vec cur, red;
+                Vec cur, red;
 #pragma unroll
-            for (int j = 0; j < KMAX; ++j) {
-                cur.a[j] = V(0);
-                red.a[j] = V(0);
-            }
-            bool has = false;
-            int seg = y_off - 1;
-            auto step = [&](int r, V vr, uint32_t cr) {
-                if ((bits >> r) & 1u) {
-                    if (has)
-                        emit(seg, cur);
-                    else
-                        red = cur;
-                    has = true;
-                    ++seg;
-#pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        cur.a[j] = V(0);
+                for (int i = 0; i < GG; ++i) {
+                    cur.a[i] = V(0);
+                    red.a[i] = V(0);
                 }
+                bool has = false;
+                int seg = y_off - 1;
+                auto step = [&](int r, V vr, uint32_t cr) {
+                    if ((bits >> r) & 1u) {
+                        if (has)
+                            emit(seg, cur);
+                        else
+                            red = cur;
+                        has = true;
+                        ++seg;
+#pragma unroll
+                        for (int i = 0; i < GG; ++i)
+                            cur.a[i] = V(0);
+                    }
+                    if constexpr (XREG) {
+                        cur.a[0] = fma(vr, xg[r], cur.a[0]); // padded steps hold x = 0
+                    } else if (!((pad >> r) & 1u)) {
+                        const V* xr = x + static_cast<size_t>(cr) * rs + static_cast<size_t>(g) * xcs;
+#pragma unroll
+                        for (int i = 0; i < GG; ++i)
+                            if (i < kg)
+                                cur.a[i] = fma(vr, xr[i * xcs], cur.a[i]);
+                    }
+                };
                 if constexpr (REG) {
-                    cur.a[0] = fma(vr, xg[r], cur.a[0]); // padded steps hold x = 0
-                } else if (!((pad >> r) & 1u)) {
-                    const V* xr = x + static_cast<size_t>(cr) * rs;
 #pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        if (j < k)
-                            cur.a[j] = fma(vr, xr[j * cs], cur.a[j]);
-                }
-            };
-            if constexpr (REG) {
-#pragma unroll
-                for (int r = 0; r < SIGMA; ++r)
-                    step(r, cu.v[r], 0u);
-            } else {
+                    for (int r = 0; r < SIGMA; ++r)
+                        step(r, cu.v[r], XREG ? 0u : cu.col[r]);
+                } else {
 #pragma unroll 1
-                for (int r = 0; r < SIGMA; ++r)
-                    step(r, ldg_stream<!DOT>(vals + r * 64), ldg_stream<!DOT>(cols + r * 64));
-            }
-
-            V blue[KMAX];
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j) {
-                if (has) {
-                    blue[j] = cur.a[j];
-                } else {
-                    red.a[j] = cur.a[j];
-                    blue[j] = V(0);
+                    for (int r = 0; r < SIGMA; ++r)
+                        step(r, ldg_stream<!DOT>(vals + r * 64), ldg_stream<!DOT>(cols + r * 64));
                 }
-            }
 
-            // A lane in which a row starts closes the row left open by the
-            // previous such lane: that lane's trailing (blue) partial plus the
-            // leading (red) partials of the lanes in between, summed by a
-            // segmented inclusive scan (a new segment begins after every lane
-            // in which a row starts). When every lane starts a row (rows no
-            // longer than sigma, the common case) the scan is the identity and
-            // the previous such lane is the lane below: one DPP shift.
-            const bool all_lanes = started == ~0ull;
-            V A[KMAX], bc[KMAX];
+                V blue[GG];
 #pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                A[j] = red.a[j];
-            if (all_lanes) {
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    bc[j] = (j < k) ? dpp_val<kDppWaveShr1, 0xf>(blue[j]) : V(0);
-            } else {
-                int f = (lane == 0) ? 1 : static_cast<int>((started >> (lane - 1)) & 1ull);
-                wave_seg_scan<V, KMAX>(A, f, k);
-                const uint64_t below = lane ? (started & ((1ull << lane) - 1ull)) : 0ull;
-                const int cprev = below ? 63 - __clzll(below) : 0;
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    bc[j] = __shfl(blue[j], cprev, 64);
-            }
-            if (lane > 0 && has) {
-                Vec tot;
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    tot.a[j] = bc[j] + A[j];
-                emit(y_off - 1, tot);
-            }
-
-            if (!starts_row && nseg > 1) {
-                // exactly one lane closed segment 0
-                const int o = __ffsll(static_cast<long long>(__ballot(own0))) - 1;
-                Vec s0;
-#pragma unroll
-                for (int j = 0; j < KMAX; ++j)
-                    s0.a[j] = (j < k) ? lane_value(seg0.a[j], o) : V(0);
-                if (lane == 0)
-                    close_carried(s0);
-            }
-
-            // The tile's last segment, known to every lane.
-            const int L = all_lanes ? 63 : 63 - __clzll(started);
-            Vec tv;
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j) {
-                if (j < k) {
-                    const V bl = lane_value(blue[j], L);
-                    const V a63 = lane_value(A[j], 63);
-                    tv.a[j] = (L == 63) ? bl : bl + a63;
-                } else {
-                    tv.a[j] = V(0);
+                for (int i = 0; i < GG; ++i) {
+                    if (has) {
+                        blue[i] = cur.a[i];
+                    } else {
+                        red.a[i] = cur.a[i];
+                        blue[i] = V(0);
+                    }
                 }
-            }
-            if (nseg == 1 && !starts_row) {
-                // the whole tile continues the carried row
-                if (complete) {
+
+                // A lane in which a row starts closes the row left open by the
+                // previous such lane: that lane's trailing (blue) partial plus
+                // the leading (red) partials of the lanes in between, summed by
+                // a segmented inclusive scan (a new segment begins after every
+                // lane in which a row starts). When every lane starts a row
+                // (rows no longer than sigma, the common case) the scan is the
+                // identity and the previous such lane is the lane below: one
+                // DPP shift.
+                const bool all_lanes = started == ~0ull;
+                V A[GG], bc[GG];
+#pragma unroll
+                for (int i = 0; i < GG; ++i)
+                    A[i] = red.a[i];
+                if (all_lanes) {
+#pragma unroll
+                    for (int i = 0; i < GG; ++i)
+                        bc[i] = (i < kg) ? dpp_val<kDppWaveShr1, 0xf>(blue[i]) : V(0);
+                } else {
+                    int f = (lane == 0) ? 1 : static_cast<int>((started >> (lane - 1)) & 1ull);
+                    wave_seg_scan<V, GG>(A, f, kg);
+                    const uint64_t below = lane ? (started & ((1ull << lane) - 1ull)) : 0ull;
+                    const int cprev = below ? 63 - __clzll(below) : 0;
+#pragma unroll
+                    for (int i = 0; i < GG; ++i)
+                        bc[i] = __shfl(blue[i], cprev, 64);
+                }
+                if (lane > 0 && has) {
+                    Vec tot;
+#pragma unroll
+                    for (int i = 0; i < GG; ++i)
+                        tot.a[i] = bc[i] + A[i];
+                    emit(y_off - 1, tot);
+                }
+
+                if (!starts_row && nseg > 1) {
+                    // exactly one lane closed segment 0
+                    const int o = __ffsll(static_cast<long long>(__ballot(own0))) - 1;
+                    Vec s0;
+#pragma unroll
+                    for (int i = 0; i < GG; ++i)
+                        s0.a[i] = (i < kg) ? lane_value(seg0.a[i], o) : V(0);
                     if (lane == 0)
-                        close_carried(tv);
-#pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        carry[j] = V(0);
-                    carry_is_head = false;
-                } else {
-#pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        carry[j] = carry[j] + tv.a[j];
+                        close_carried(s0);
                 }
-            } else {
-                if (!starts_row)
-                    carry_is_head = false; // segment 0 closed the carried row above
-                if (complete) {
-                    if (lane == 63)
-                        emit(nseg - 1, tv);
+
+                // The tile's last segment, known to every lane.
+                const int L = all_lanes ? 63 : 63 - __clzll(started);
+                Vec tv;
 #pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        carry[j] = V(0);
+                for (int i = 0; i < GG; ++i) {
+                    if (i < kg) {
+                        const V bl = lane_value(blue[i], L);
+                        const V a63 = lane_value(A[i], 63);
+                        tv.a[i] = (L == 63) ? bl : bl + a63;
+                    } else {
+                        tv.a[i] = V(0);
+                    }
+                }
+                if (nseg == 1 && !starts_row) {
+                    // the whole tile continues the carried row
+                    if (complete) {
+                        if (lane == 0)
+                            close_carried(tv);
+#pragma unroll
+                        for (int i = 0; i < GG; ++i)
+                            carry[g + i] = V(0);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < GG; ++i)
+                            carry[g + i] = carry[g + i] + tv.a[i];
+                    }
                 } else {
+                    if (complete) {
+                        if (lane == 63)
+                            emit(nseg - 1, tv);
 #pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        carry[j] = tv.a[j];
+                        for (int i = 0; i < GG; ++i)
+                            carry[g + i] = V(0);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < GG; ++i)
+                            carry[g + i] = tv.a[i];
+                    }
                 }
             }
+            // segment 0 closed the carried row, or the whole tile did
+            if (!starts_row && (nseg != 1 || complete))
+                carry_is_head = false;
         };
         if (dirty)
             process(std::true_type{});
@@ -688,7 +763,7 @@ struct FixParams {
     uint32_t tile_begin, row_base, empty_begin, empty_end;
     uint32_t tpw, nchunks;
     int k;
-    uint64_t rs, cs;             // as TileParams::rs, cs
+    uint64_t rs, xcs, ycs;       // as TileParams::rs, xcs, ycs
     uint32_t dot_ld, dot_off;    // dot partial of column j goes to dot_blk[block*dot_ld + dot_off + j]
 };
 
@@ -705,7 +780,8 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
     const V* x = static_cast<const V*>(p.x);
     V* y = static_cast<V*>(p.y);
     const size_t rs = KMAX == 1 ? 1 : p.rs;
-    const size_t cs = KMAX == 1 ? 0 : p.cs;
+    const size_t xcs = KMAX == 1 ? 0 : p.xcs;
+    const size_t ycs = KMAX == 1 ? 0 : p.ycs;
     const V* head = static_cast<const V*>(p.head);
     const V* tail = static_cast<const V*>(p.tail);
     double dacc[KMAX];
@@ -735,9 +811,9 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
                     } else {
                         s = head[static_cast<size_t>(c) * k + j];
                     }
-                    yr[j * cs] = s;
+                    yr[j * ycs] = s;
                     if constexpr (DOT)
-                        dacc[j] += static_cast<double>(x[static_cast<size_t>(row) * rs + j * cs]) * static_cast<double>(s);
+                        dacc[j] += static_cast<double>(x[static_cast<size_t>(row) * rs + j * xcs]) * static_cast<double>(s);
                 }
             }
         }
@@ -754,7 +830,7 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
         const uint32_t row = p.empty_rows[p.empty_begin + i];
         V* yr = y + static_cast<size_t>(row - p.row_base) * rs;
         for (int j = 0; j < k; ++j)
-            yr[j * cs] = V(0);
+            yr[j * ycs] = V(0);
     }
 
     if constexpr (DOT) {

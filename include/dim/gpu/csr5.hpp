@@ -125,6 +125,11 @@ class csr5 {
     void spmm(uint32_t k, const double* X, double* Y, void* stream = nullptr) const {
         check(dim_csr5_spmm(h_.get(), DIM_F64, k, X, Y, stream));
     }
+    // the same on column-major blocks with leading dimensions ldx, ldy
+    void spmm_colmajor(uint32_t k, const double* X, uint64_t ldx, double* Y, uint64_t ldy,
+                       void* stream = nullptr) const {
+        check(dim_csr5_spmm_colmajor(h_.get(), DIM_F64, k, X, ldx, Y, ldy, stream));
+    }
 
     dimensions_t dimensions() const {
         const auto i = info();

@@ -125,6 +125,13 @@ dim_status dim_csr5_spmv_block(const dim_csr5* A, dim_dtype dtype, uint32_t tile
  * single-vector spmv). */
 dim_status dim_csr5_spmm(const dim_csr5* A, dim_dtype dtype, uint32_t k, const void* X, void* Y, void* stream);
 
+/* The same product on column-major blocks: entry (i, j) of X at X[i + j*ldx]
+ * (ldx >= cols) and of Y at Y[i + j*ldy] (ldy >= rows). The faster layout:
+ * each right-hand side is gathered from its own contiguous column. No
+ * reference counterpart. */
+dim_status dim_csr5_spmm_colmajor(const dim_csr5* A, dim_dtype dtype, uint32_t k, const void* X, uint64_t ldx,
+                                  void* Y, uint64_t ldy, void* stream);
+
 /* y = A^T x (x: rows entries, y: cols entries). No reference counterpart;
  * used by the adjoint. */
 dim_status dim_csr5_spmv_transpose(const dim_csr5* A, dim_dtype dtype, const void* x, void* y, void* stream);

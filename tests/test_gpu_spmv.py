@@ -160,6 +160,40 @@ def test_spmm_equals_columnwise_spmv(cuda, k, sigma):
     np.testing.assert_array_equal(Y32, Y)  # small integers are exact in FP32 too
 
 
+@pytest.mark.parametrize("k", [2, 3, 8, 13, 16])
+@pytest.mark.parametrize("sigma", [4, 7, 8])
+@pytest.mark.parametrize("layout", ["row", "col", "col_ld"])
+def test_spmm_short_rows_both_layouts(cuda, k, sigma, layout):
+    """Tiles of short rows (the 7-point operator on a 40^3 grid) take the
+    per-side fast path for sigma 7 and 8 and a mix with the general path for
+    sigma 4; integer inputs make FP64 and FP32 exact. Column-major blocks, also
+    with a leading dimension beyond the column length, give column-major Y."""
+    import torch
+
+    side = 40
+    rp, ci, v = matgen.poisson3d(side)
+    n = side ** 3
+    X = np.random.default_rng(k * 100 + sigma).integers(-4, 5, (n, k)).astype(np.float64)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=sigma, keep_f32=True)
+    want = np.stack([O.csr_spmv(rp, ci, v, X[:, j].copy()) for j in range(k)], 1)
+    for dt in (torch.float64, torch.float32):
+        Xd = to_dev(X, dt)
+        if layout == "col":
+            Xd = Xd.t().contiguous().t()
+        elif layout == "col_ld":
+            buf = torch.full((k, n + 7), float("nan"), dtype=dt, device="cuda")
+            buf[:, :n] = Xd.t()
+            Xd = buf[:, :n].t()
+        Y = A.spmm(Xd)
+        assert (Y.stride(0) == 1) == (layout != "row")
+        np.testing.assert_array_equal(Y.double().cpu().numpy(), want)
+    if layout == "col_ld":
+        Ybuf = torch.full((k, n + 3), -1.0, dtype=torch.float64, device="cuda")
+        A.spmm(to_dev(X).t().contiguous().t(), Ybuf[:, :n].t())
+        np.testing.assert_array_equal(Ybuf[:, :n].t().cpu().numpy(), want)
+        assert (Ybuf[:, n:] == -1.0).all()  # nothing written past the column length
+
+
 @pytest.mark.parametrize("sigma", SIGMAS)
 def test_transpose_product(cuda, sigma):
     rp, ci, v = matgen.random_csr(2000, 1700, seed=sigma, max_len=15, empty_frac=0.1, long_rows=[(2, 1600)],
