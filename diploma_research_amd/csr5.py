@@ -143,10 +143,12 @@ class Csr5:
 
     # ---------------------------------------------------------- creation --
     @classmethod
-    def from_csr(cls, csr, sigma: int = 16, keep_f32: bool = False) -> "Csr5":
+    def from_csr(cls, csr, sigma: int = 16, keep_f32: bool = False, xcd_slices: bool = False) -> "Csr5":
         """csr5::from_csr (csr5.h:791-803). `csr` is a Csr or a (rows, cols,
         row_ptr, col_idx, vals) tuple of host arrays. sigma=0 runs the
-        autotuner (times the candidate sigmas on this matrix)."""
+        autotuner (times the candidate sigmas on this matrix, and the
+        column-sliced form for large scattered x). xcd_slices=True forces the
+        column-sliced form for spmv."""
         if isinstance(csr, Csr):
             rows, cols = csr.dimensions.rows, csr.dimensions.cols
             rp, ci, v = csr.row_start_offsets, csr.col_indices, csr.values
@@ -157,7 +159,8 @@ class Csr5:
         v = np.ascontiguousarray(v, dtype=np.float64)
         if len(rp) != rows + 1:
             raise ValueError("row_ptr must have rows + 1 entries")
-        opts = N.Csr5Opts(sigma, N.DIM_CSR5_KEEP_F32 if keep_f32 else 0)
+        flags = (N.DIM_CSR5_KEEP_F32 if keep_f32 else 0) | (N.DIM_CSR5_XCD_SLICES if xcd_slices else 0)
+        opts = N.Csr5Opts(sigma, flags)
         h = C.c_void_p()
         N.check(N.lib().dim_csr5_from_csr(rows, cols, len(v), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
                                           C.byref(opts), C.byref(h)))

@@ -39,6 +39,7 @@ void free_handle(dim_csr5* h) {
     for (void* p : h->allocations)
         (void)hipFree(p);
     (void)hipSetDevice(prev);
+    free_handle(h->xs);
     delete h;
 }
 
@@ -155,11 +156,126 @@ dim_status check_handle(const dim_csr5* A) {
     return DIM_SUCCESS;
 }
 
+// ---- Column slices ---------------------------------------------------------
+// A product that gathers x entries from all over a large x (random columns)
+// is bound by the rate of L2 misses, not by HBM: every XCD's L2 (4 MB) holds a
+// small share of x. Splitting the columns into kXcdSlices slices of equal
+// element count and stacking the slices as row blocks (row s*rows + i of the
+// stacked matrix holds the elements of row i whose columns lie in slice s, in
+// CSR order) lets the tile kernel, whose XCD remap gives each XCD one
+// contiguous eighth of the tiles, gather each XCD's elements from about one
+// eighth of x, which then stays in that XCD's L2. k_slice_sum adds the
+// per-slice partial products of each row in slice order. Measured on the
+// MI355X: 62M gathers from 32 MB take 1.01 ms at random and 0.26 ms sliced.
+constexpr uint32_t kXcdSlices = 8;
+
+// The stacked matrix keeps only the (slice, row) pairs that hold elements, so
+// that it has no empty rows (and no dirty tiles): stacked row k is the k-th
+// set bit of `present`, bit s*rows + i. rank[w] counts the set bits in the
+// words before word w.
+struct StackedCsr {
+    uint32_t rows = 0;
+    std::vector<uint32_t> row_ptr, col_idx;
+    std::vector<double> vals;
+    std::vector<uint32_t> present, rank;
+};
+
+// x far beyond one L2, rows long enough that one partial per slice and row
+// costs less than the gathers it localises, and row indices that still fit
+bool slices_eligible(uint32_t rows, uint32_t cols, uint64_t nnz) {
+    return uint64_t{cols} * sizeof(double) >= (16ull << 20) && nnz >= uint64_t{kXcdSlices} * rows &&
+           uint64_t{rows} * kXcdSlices < (1ull << 31);
+}
+
+StackedCsr stack_slices(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr, const uint32_t* col_idx,
+                        const double* vals, uint32_t S) {
+    // column cuts at equal element counts: slice of column c = number of cuts <= c
+    std::vector<uint64_t> below(size_t{cols} + 1, 0); // below[c]: elements with column < c
+    for (uint64_t e = 0; e < nnz; ++e)
+        ++below[size_t{col_idx[e]} + 1];
+    for (size_t c = 1; c <= cols; ++c)
+        below[c] += below[c - 1];
+    std::vector<uint8_t> slice_of(cols, 0);
+    {
+        uint32_t q = 0;
+        for (uint32_t c = 0; c < cols; ++c) {
+            while (q + 1 < S && below[c] * S >= nnz * (q + 1) && nnz)
+                ++q;
+            slice_of[c] = static_cast<uint8_t>(q);
+        }
+    }
+    const size_t pairs = size_t{S} * rows;
+    std::vector<uint32_t> count(pairs, 0); // elements of (slice, row), slice-major
+    for (uint32_t i = 0; i < rows; ++i)
+        for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+            ++count[size_t{slice_of[col_idx[e]]} * rows + i];
+    StackedCsr out;
+    out.present.assign((pairs + 31) / 32, 0);
+    out.rank.assign(out.present.size(), 0);
+    out.row_ptr.push_back(0);
+    std::vector<uint32_t> at(pairs, 0); // next element slot of each pair
+    for (size_t r = 0; r < pairs; ++r) {
+        if (!(r & 31))
+            out.rank[r >> 5] = out.rows;
+        if (count[r]) {
+            out.present[r >> 5] |= 1u << (r & 31);
+            at[r] = out.row_ptr.back();
+            out.row_ptr.push_back(out.row_ptr.back() + count[r]);
+            ++out.rows;
+        }
+    }
+    out.col_idx.resize(nnz);
+    out.vals.resize(nnz);
+    for (uint32_t i = 0; i < rows; ++i)
+        for (uint32_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+            const uint32_t k = at[size_t{slice_of[col_idx[e]]} * rows + i]++;
+            out.col_idx[k] = col_idx[e];
+            out.vals[k] = vals[e];
+        }
+    return out;
+}
+
+// Builds the stacked matrix's format with tile height sigma, with its
+// partial-product buffer and the bit map (and word ranks) of the pairs.
+dim_status build_slices(const StackedCsr& sc, uint32_t cols, uint32_t sigma, bool keep_f32, dim_csr5** out) {
+    HostFormat f;
+    const std::string err = build_format(sc.rows, cols, sc.vals.size(), sc.row_ptr.data(), sc.col_idx.data(),
+                                         sc.vals.data(), sigma, f);
+    if (!err.empty())
+        return fail(DIM_INVALID_ARGUMENT, err);
+    dim_csr5* h = nullptr;
+    if (dim_status st = upload_format(f, keep_f32, &h))
+        return st;
+    std::unique_ptr<dim_csr5, void (*)(dim_csr5*)> xs(h, free_handle);
+    DIM_HIP_TRY(upload(xs.get(), xs->present, sc.present));
+    DIM_HIP_TRY(upload(xs.get(), xs->present_rank, sc.rank));
+    double* ys = nullptr;
+    DIM_HIP_TRY(dev_alloc(xs.get(), ys, sc.rows));
+    xs->ystack = ys;
+    *out = xs.release();
+    return DIM_SUCCESS;
+}
+
+// The whole-matrix single-vector product, through the column slices when the
+// matrix has them.
+hipError_t plain_product(dim_csr5* h, bool f32, const void* x, void* y, hipStream_t st) {
+    if (h->xs) {
+        dim_csr5* xs = h->xs;
+        hipError_t e = launch_spmv(xs->f, xs->s, f32, 1, x, xs->ystack, 0, xs->f.tiles, 0, 0, xs->f.n_empty, false,
+                                   nullptr, st);
+        if (e != hipSuccess)
+            return e;
+        return launch_slice_sum(f32, xs->ystack, xs->present, xs->present_rank, h->f.rows, h->xs_slices, y, st);
+    }
+    return launch_spmv(h->f, h->s, f32, 1, x, y, 0, h->f.tiles, 0, 0, h->f.n_empty, false, nullptr, st);
+}
+
 // Times candidate tile heights on this matrix and keeps the fastest:
 // sigma trades per-tile fixed work (descriptor decode, wavefront scan, carry
 // slots) against registers per lane and load-balance granularity.
 dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
-                          const uint32_t* col_idx, const double* vals, bool keep_f32, dim_csr5** out) {
+                          const uint32_t* col_idx, const double* vals, bool keep_f32, bool force_slices,
+                          dim_csr5** out) {
     const std::vector<uint32_t> candidates = sigma_candidates(rows, nnz);
     dim_csr5* best = nullptr;
     struct Tried {
@@ -185,6 +301,26 @@ dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
+    // best of three rounds of ten products: the rounds absorb clock and queue
+    // noise that a single round shows
+    auto time_products = [&](dim_csr5* h, float* ms) -> dim_status {
+        for (int i = 0; i < 3; ++i)
+            (void)plain_product(h, false, x, y, 0);
+        const int reps = 10;
+        hipError_t e = hipSuccess;
+        for (int round = 0; round < 3 && e == hipSuccess; ++round) {
+            (void)hipEventRecord(e0, 0);
+            for (int i = 0; i < reps; ++i)
+                (void)plain_product(h, false, x, y, 0);
+            (void)hipEventRecord(e1, 0);
+            e = hipEventSynchronize(e1);
+            float r = 0;
+            (void)hipEventElapsedTime(&r, e0, e1);
+            if (round == 0 || r < *ms)
+                *ms = r;
+        }
+        return e == hipSuccess ? DIM_SUCCESS : hip_fail(e, "autotune timing");
+    };
     dim_status st = DIM_SUCCESS;
     for (uint32_t sigma : candidates) {
         HostFormat f;
@@ -197,31 +333,44 @@ dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint
         st = upload_format(f, keep_f32, &h);
         if (st != DIM_SUCCESS)
             break;
-        for (int i = 0; i < 3; ++i)
-            (void)launch_spmv(h->f, h->s, false, 1, x, y, 0, h->f.tiles, 0, 0, h->f.n_empty, false, nullptr, 0);
-        // best of three rounds of ten products: the rounds absorb clock and
-        // queue noise that a single round shows
-        const int reps = 10;
         float ms = 0;
-        hipError_t e = hipSuccess;
-        for (int round = 0; round < 3 && e == hipSuccess; ++round) {
-            (void)hipEventRecord(e0, 0);
-            for (int i = 0; i < reps; ++i)
-                (void)launch_spmv(h->f, h->s, false, 1, x, y, 0, h->f.tiles, 0, 0, h->f.n_empty, false, nullptr,
-                                  0);
-            (void)hipEventRecord(e1, 0);
-            e = hipEventSynchronize(e1);
-            float r = 0;
-            (void)hipEventElapsedTime(&r, e0, e1);
-            if (round == 0 || r < ms)
-                ms = r;
-        }
-        if (e != hipSuccess) {
+        st = time_products(h, &ms);
+        if (st != DIM_SUCCESS) {
             free_handle(h);
-            st = hip_fail(e, "autotune timing");
             break;
         }
         tried.push_back({h, ms});
+    }
+    // Column slices: the stacked matrix at the tile heights near its (short)
+    // mean row length, timed as whole products through the first candidate.
+    dim_csr5* xs_best = nullptr;
+    float xs_ms = 0;
+    if (st == DIM_SUCCESS && !tried.empty() && nnz && (force_slices || slices_eligible(rows, cols, nnz))) {
+        const StackedCsr sc = stack_slices(rows, cols, nnz, row_ptr, col_idx, vals, kXcdSlices);
+        dim_csr5* probe = tried[0].h;
+        for (uint32_t sigma : {4u, 8u, 16u}) {
+            dim_csr5* xs = nullptr;
+            st = build_slices(sc, cols, sigma, keep_f32, &xs);
+            if (st != DIM_SUCCESS)
+                break;
+            probe->xs = xs;
+            probe->xs_slices = kXcdSlices;
+            float ms = 0;
+            st = time_products(probe, &ms);
+            probe->xs = nullptr;
+            probe->xs_slices = 0;
+            if (st != DIM_SUCCESS) {
+                free_handle(xs);
+                break;
+            }
+            if (!xs_best || ms < xs_ms) {
+                free_handle(xs_best);
+                xs_best = xs;
+                xs_ms = ms;
+            } else {
+                free_handle(xs);
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -242,6 +391,17 @@ dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint
                 best = c.h;
         }
     }
+    if (st == DIM_SUCCESS && best && xs_best) {
+        float fastest = tried[0].ms;
+        for (const auto& c : tried)
+            fastest = std::min(fastest, c.ms);
+        if (force_slices || xs_ms < 0.97f * fastest) {
+            best->xs = xs_best;
+            best->xs_slices = kXcdSlices;
+            xs_best = nullptr;
+        }
+    }
+    free_handle(xs_best);
     for (const auto& c : tried)
         if (c.h != best)
             free_handle(c.h);
@@ -291,13 +451,28 @@ dim_status dim_csr5_from_csr(uint32_t rows, uint32_t cols, uint64_t nnz, const u
         return fail(DIM_INVALID_ARGUMENT, err);
     const uint32_t sigma = opts ? opts->sigma : 16;
     const bool keep_f32 = opts && (opts->flags & DIM_CSR5_KEEP_F32);
+    const bool slices = opts && (opts->flags & DIM_CSR5_XCD_SLICES);
+    if (slices && uint64_t{rows} * kXcdSlices >= (1ull << 31))
+        return fail(DIM_INVALID_ARGUMENT, "DIM_CSR5_XCD_SLICES needs rows < 2^28");
     if (sigma == 0)
-        return autotune_sigma(rows, cols, nnz, row_ptr, col_idx, vals, keep_f32, out);
+        return autotune_sigma(rows, cols, nnz, row_ptr, col_idx, vals, keep_f32, slices, out);
     HostFormat f;
     err = build_format(rows, cols, nnz, row_ptr, col_idx, vals, sigma, f);
     if (!err.empty())
         return fail(DIM_INVALID_ARGUMENT, err);
-    return upload_format(f, keep_f32, out);
+    dim_csr5* h = nullptr;
+    if (dim_status st = upload_format(f, keep_f32, &h))
+        return st;
+    if (slices && nnz) {
+        const StackedCsr sc = stack_slices(rows, cols, nnz, row_ptr, col_idx, vals, kXcdSlices);
+        if (dim_status st = build_slices(sc, cols, sigma, keep_f32, &h->xs)) {
+            free_handle(h);
+            return st;
+        }
+        h->xs_slices = kXcdSlices;
+    }
+    *out = h;
+    return DIM_SUCCESS;
 }
 
 dim_status dim_csr5_destroy(dim_csr5* A) {
@@ -320,7 +495,9 @@ dim_status dim_csr5_get_info(const dim_csr5* A, dim_csr5_info* info) {
     info->n_empty_rows = A->f.n_empty;
     info->n_dirty_tiles = A->n_dirty;
     info->has_f32 = A->f.vals32 != nullptr;
-    info->device_bytes = A->device_bytes;
+    info->device_bytes = A->device_bytes + (A->xs ? A->xs->device_bytes : 0);
+    info->xcd_slices = A->xs ? A->xs_slices : 0;
+    info->xcd_sigma = A->xs ? A->xs->f.sigma : 0;
     return DIM_SUCCESS;
 }
 
@@ -378,8 +555,7 @@ dim_status dim_csr5_spmv(const dim_csr5* A, dim_dtype dtype, const void* x, uint
     if ((A->f.cols && !x) || (A->f.rows && !y))
         return fail(DIM_INVALID_ARGUMENT, "null vector");
     auto* self = const_cast<dim_csr5*>(A);
-    DIM_HIP_TRY(launch_spmv(self->f, self->s, dtype == DIM_F32, 1, x, y, 0, A->f.tiles, 0, 0, A->f.n_empty, false,
-                            nullptr, static_cast<hipStream_t>(stream)));
+    DIM_HIP_TRY(plain_product(self, dtype == DIM_F32, x, y, static_cast<hipStream_t>(stream)));
     return DIM_SUCCESS;
 }
 
@@ -396,8 +572,22 @@ dim_status dim_csr5_spmv_timed(const dim_csr5* A, dim_dtype dtype, const void* x
         DIM_HIP_TRY(hipEventCreate(&e));
     auto* self = const_cast<dim_csr5*>(A);
     bool fixup = true;
-    hipError_t e = launch_spmv_events(self->f, self->s, dtype == DIM_F32, x, y, ev[0], ev[1], ev[2],
-                                      static_cast<hipStream_t>(stream), &fixup);
+    hipError_t e;
+    if (self->xs) {
+        // the stacked matrix's tile kernel; its fix-up (if any) and the slice sum
+        const auto st = static_cast<hipStream_t>(stream);
+        e = launch_spmv_events(self->xs->f, self->xs->s, dtype == DIM_F32, x, self->xs->ystack, ev[0], ev[1], ev[2],
+                               st, &fixup);
+        if (e == hipSuccess)
+            e = launch_slice_sum(dtype == DIM_F32, self->xs->ystack, self->xs->present, self->xs->present_rank,
+                                 self->f.rows, self->xs_slices, y, st);
+        if (e == hipSuccess)
+            e = hipEventRecord(ev[2], st);
+        fixup = true;
+    } else {
+        e = launch_spmv_events(self->f, self->s, dtype == DIM_F32, x, y, ev[0], ev[1], ev[2],
+                               static_cast<hipStream_t>(stream), &fixup);
+    }
     if (e == hipSuccess)
         e = hipEventSynchronize(ev[2]);
     if (e == hipSuccess)
@@ -529,6 +719,11 @@ dim_status dim_csr5_set_values(dim_csr5* A, const double* vals, void* stream) {
         return st;
     if (A->nnz_logical && !vals)
         return fail(DIM_INVALID_ARGUMENT, "vals is null");
+    // the column-sliced copy would go stale: products fall back to the
+    // unsliced form (hipFree waits for work in flight)
+    free_handle(A->xs);
+    A->xs = nullptr;
+    A->xs_slices = 0;
     for (uint32_t s = 0; s < A->n_systems; ++s) {
         const DevFormat view = system_view(A, s);
         DIM_HIP_TRY(launch_set_values(view, vals + A->sys_nnz_csr[s], static_cast<hipStream_t>(stream)));

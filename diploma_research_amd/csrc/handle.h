@@ -25,6 +25,17 @@ struct dim_csr5 {
     std::vector<uint64_t> sys_nnz_csr;
     uint64_t nnz_logical = 0; // stored elements (the format's nnz counts padding inside a batch)
     dimgpu::BoundaryInfo binfo; // host copy of the tile flags (f.binfo points here)
+    // Column slices for plain products (capi.cpp): xs is the matrix whose
+    // rows are the (column slice s, row i) pairs that hold elements, in the
+    // order of s*rows + i. That stacked handle also owns the partial products
+    // (ystack, one FP64-sized entry per stacked row), the bit map of the
+    // pairs (present, bit s*rows + i) and the set bits before each word of
+    // it (present_rank).
+    dim_csr5* xs = nullptr;
+    uint32_t xs_slices = 0;
+    void* ystack = nullptr;
+    uint32_t* present = nullptr;
+    uint32_t* present_rank = nullptr;
 };
 
 namespace dimgpu {

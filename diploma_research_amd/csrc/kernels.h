@@ -88,6 +88,13 @@ hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const vo
 hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, void* y, hipStream_t stream,
                                  const int* done = nullptr);
 
+// y[i] = sum over the column slices s = 0..slices-1, in that order, of the
+// partial product of (s, i): the partials of the pairs whose bit s*rows + i
+// is set in `present`, stored at the rank of that bit (rank[w] = set bits
+// before word w); the other pairs count as zero.
+hipError_t launch_slice_sum(bool f32, const void* ystack, const uint32_t* present, const uint32_t* rank,
+                            uint32_t rows, uint32_t slices, void* y, hipStream_t stream);
+
 // Scatters CSR-order values into the tile layout (FP64, and FP32 if kept).
 hipError_t launch_set_values(const DevFormat& A, const double* vals_csr, hipStream_t stream);
 

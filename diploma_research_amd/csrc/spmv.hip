@@ -20,6 +20,8 @@
 //     reproducible from run to run.
 #include "spmv_impl.h"
 
+#include <algorithm>
+
 namespace dimgpu {
 
 namespace {
@@ -294,6 +296,57 @@ hipError_t launch_spmv_events(const DevFormat& A, Scratch& S, bool f32, const vo
     e = hipEventRecord(e2, stream);
     if (e != hipSuccess)
         return e;
+    return hipGetLastError();
+}
+
+namespace {
+
+// One row per thread; the S slices' bit words, ranks and partials are each
+// loaded together (all in flight at once) before the in-order sum.
+template<typename V, int S>
+__global__ __launch_bounds__(256) void k_slice_sum(const V* __restrict__ ys, const uint32_t* __restrict__ present,
+                                                   const uint32_t* __restrict__ rank, uint32_t rows,
+                                                   V* __restrict__ y) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= rows)
+        return;
+    uint32_t w[S], at[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const size_t r = static_cast<size_t>(q) * rows + i;
+        w[q] = present[r >> 5];
+        at[q] = rank[r >> 5];
+    }
+    V p[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const uint32_t b = static_cast<uint32_t>((static_cast<size_t>(q) * rows + i) & 31u);
+        const bool has = (w[q] >> b) & 1u;
+        p[q] = has ? ys[at[q] + __popc(w[q] & ((1u << b) - 1u))] : V(0);
+    }
+    V s = V(0);
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+        if ((w[q] >> ((static_cast<size_t>(q) * rows + i) & 31u)) & 1u)
+            s = s + p[q];
+    y[i] = s;
+}
+
+} // namespace
+
+hipError_t launch_slice_sum(bool f32, const void* ystack, const uint32_t* present, const uint32_t* rank,
+                            uint32_t rows, uint32_t slices, void* y, hipStream_t stream) {
+    if (!rows)
+        return hipSuccess;
+    if (slices != 8)
+        return hipErrorInvalidValue; // the library builds 8 slices (one per XCD)
+    const uint32_t grid = (rows + 255) / 256;
+    if (f32)
+        hipLaunchKernelGGL((k_slice_sum<float, 8>), dim3(grid), dim3(256), 0, stream,
+                           static_cast<const float*>(ystack), present, rank, rows, static_cast<float*>(y));
+    else
+        hipLaunchKernelGGL((k_slice_sum<double, 8>), dim3(grid), dim3(256), 0, stream,
+                           static_cast<const double*>(ystack), present, rank, rows, static_cast<double*>(y));
     return hipGetLastError();
 }
 

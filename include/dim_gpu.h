@@ -52,6 +52,10 @@ typedef struct dim_csr5_opts {
     uint32_t flags; /* DIM_CSR5_* below */
 } dim_csr5_opts;
 #define DIM_CSR5_KEEP_F32 1u /* also keep FP32 values (for DIM_F32 products and mixed-precision CG) */
+#define DIM_CSR5_XCD_SLICES 2u /* also keep the column-sliced form for dim_csr5_spmv (scattered columns,
+                                  large x: each XCD gathers from one eighth of x). With sigma = 0 the
+                                  autotuner adds it on its own when it is faster, and the flag forces
+                                  it. No reference counterpart. */
 
 typedef struct dim_csr5_info {
     uint32_t rows, cols;
@@ -63,6 +67,8 @@ typedef struct dim_csr5_info {
     uint32_t n_dirty_tiles;
     uint32_t has_f32;
     uint64_t device_bytes;
+    uint32_t xcd_slices; /* column slices used by dim_csr5_spmv (0: none) */
+    uint32_t xcd_sigma;  /* tile height of the sliced form */
 } dim_csr5_info;
 
 const char* dim_last_error(void);
@@ -101,7 +107,9 @@ dim_status dim_csr5_spmv(const dim_csr5* A, dim_dtype dtype, const void* x, uint
 /* Measurement variant of dim_csr5_spmv: records HIP events around the tile
  * kernel and the carry/fix-up kernel on `stream`, waits for them and returns
  * both durations in milliseconds (ms[0] tiles, ms[1] fix-up; ms[1] is 0 when
- * the product needs no fix-up kernel and none was launched). No reference
+ * the product needs no fix-up kernel and none was launched). With column
+ * slices, ms[0] is the sliced tile kernel and ms[1] its fix-up plus the slice
+ * sum. No reference
  * counterpart (the reference times whole calls with a stopwatch,
  * include/dim/bench/stopwatch.h:11-22). */
 dim_status dim_csr5_spmv_timed(const dim_csr5* A, dim_dtype dtype, const void* x, uint64_t x_len, void* y,
@@ -200,8 +208,9 @@ uint32_t dim_csr5_n_systems(const dim_csr5* A);
 
 /* Replaces the stored values (same sparsity pattern) from a DEVICE FP64 array
  * in the CSR order the matrix was built from (for batched matrices: the
- * systems' arrays back to back). Keeps the FP32 copy in step when present. No
- * reference counterpart; used when differentiating with respect to A. */
+ * systems' arrays back to back). Keeps the FP32 copy in step when present and
+ * drops the column-sliced form (DIM_CSR5_XCD_SLICES). No reference
+ * counterpart; used when differentiating with respect to A. */
 dim_status dim_csr5_set_values(dim_csr5* A, const double* vals, void* stream);
 
 #ifdef __cplusplus

@@ -1,11 +1,14 @@
 """GPU parity of the CSR5 product (HIP kernels via the C ABI) against the
 oracle and the reference's golden fixtures."""
+import ctypes
+
 import numpy as np
 import pytest
 
 from conftest import load_golden
 from gpu_util import assert_spmv_close, to_dev
 from diploma_research_amd import Csr5, matgen
+from diploma_research_amd import native as N
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -275,6 +278,89 @@ def test_powerlaw_4m_full_size(cuda):
     rp, ci, v = matgen.powerlaw(n, seed=2024)
     x = np.random.default_rng(7).uniform(-1, 1, n)
     A = Csr5.from_csr((n, n, rp, ci, v), sigma=16)
+    y = A.spmv(to_dev(x)).cpu().numpy()
+    assert_spmv_close(y, O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
+
+
+def _slice_case(name):
+    if name == "powerlaw":
+        n = 200_000
+        rp, ci, v = matgen.powerlaw(n, seed=11)
+        return n, n, rp, ci, v
+    if name == "wide":
+        rp, ci, v = matgen.random_csr(5000, 300_000, seed=12, max_len=40, empty_frac=0.1, long_rows=[(7, 250_000)])
+        return 5000, 300_000, rp, ci, v
+    if name == "empty_rows":
+        rp, ci, v = matgen.random_csr(20_000, 20_000, seed=13, max_len=30, empty_frac=0.5, long_rows=[(3, 15_000)])
+        return 20_000, 20_000, rp, ci, v
+    if name == "few_cols":  # fewer columns than slices: some slices are empty
+        rp, ci, v = matgen.random_csr(3000, 5, seed=14, max_len=5, empty_frac=0.2)
+        return 3000, 5, rp, ci, v
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["powerlaw", "wide", "empty_rows", "few_cols"])
+@pytest.mark.parametrize("sigma", [4, 8, 16])
+def test_xcd_slices_match_oracle(cuda, name, sigma):
+    """The column-sliced product (the elements of each column slice stacked
+    as row blocks, partial products added per row in slice order) against the
+    oracle: FP64 within the parity gate, FP32 within its rounding bound,
+    repeat calls bitwise equal, spmv_timed equal to spmv."""
+    import torch
+
+    rows, cols, rp, ci, v = _slice_case(name)
+    x = np.random.default_rng(sigma).uniform(-1, 1, cols)
+    A = Csr5.from_csr((rows, cols, rp, ci, v), sigma=sigma, keep_f32=True, xcd_slices=True)
+    assert A.info.xcd_slices == 8 and A.info.xcd_sigma == sigma
+    ref = O.csr_spmv(rp, ci, v, x)
+    y = A.spmv(to_dev(x))
+    assert_spmv_close(y.cpu().numpy(), ref, rp, ci, v, x)
+    assert torch.equal(A.spmv(to_dev(x)), y)
+    yt = torch.full_like(y, float("nan"))
+    A.spmv_timed(to_dev(x), yt)
+    assert torch.equal(yt, y)
+    y32 = A.spmv(to_dev(x, torch.float32)).double().cpu().numpy()
+    u32 = np.finfo(np.float32).eps / 2
+    lens = np.diff(rp.astype(np.int64))
+    absum = np.bincount(np.repeat(np.arange(rows), lens), weights=np.abs(v * x[ci]), minlength=rows)
+    assert (np.abs(y32 - ref) <= (lens + 4) * u32 * absum + 1e-30).all()
+
+
+def test_xcd_slices_empty_matrix(cuda):
+    import torch
+
+    A = Csr5.from_csr((7, 5, np.zeros(8, np.uint32), np.zeros(0, np.uint32), np.zeros(0)), xcd_slices=True)
+    y = A.spmv(torch.ones(5, dtype=torch.float64, device="cuda"), torch.full((7,), 9.0, dtype=torch.float64,
+                                                                             device="cuda"))
+    assert torch.equal(y.cpu(), torch.zeros(7, dtype=torch.float64))
+
+
+def test_xcd_slices_integer_exact_and_set_values(cuda):
+    """Integer inputs: the sliced sums are exact. Replacing the values drops
+    the sliced form, and products then follow the new values."""
+    import torch
+
+    rp, ci, v = matgen.random_csr(6000, 90_000, seed=21, max_len=60, empty_frac=0.1, integer=True)
+    x = np.random.default_rng(2).integers(-3, 4, 90_000).astype(np.float64)
+    A = Csr5.from_csr((6000, 90_000, rp, ci, v), sigma=8, xcd_slices=True)
+    np.testing.assert_array_equal(A.spmv(to_dev(x)).cpu().numpy(), O.csr_spmv(rp, ci, v, x))
+    v2 = v * 2 - 1
+    A.set_values(to_dev(v2))
+    info = N.Csr5Info()
+    N.check(N.lib().dim_csr5_get_info(A._h, ctypes.byref(info)))
+    assert info.xcd_slices == 0
+    np.testing.assert_array_equal(A.spmv(to_dev(x)).cpu().numpy(), O.csr_spmv(rp, ci, v2, x))
+
+
+def test_xcd_slices_autotuned_on_powerlaw_4m(cuda):
+    """sigma = 0 on the 4M power-law matrix (x = 32 MB, random columns): the
+    autotuner weighs the sliced form against the unsliced one; the product it
+    keeps matches the oracle either way."""
+    n = 4_000_000
+    rp, ci, v = matgen.powerlaw(n, seed=2024)
+    x = np.random.default_rng(7).uniform(-1, 1, n)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=0)
+    print("xcd_slices", A.info.xcd_slices, "xcd_sigma", A.info.xcd_sigma, "sigma", A.sigma)
     y = A.spmv(to_dev(x)).cpu().numpy()
     assert_spmv_close(y, O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
 
