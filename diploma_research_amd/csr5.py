@@ -143,12 +143,14 @@ class Csr5:
 
     # ---------------------------------------------------------- creation --
     @classmethod
-    def from_csr(cls, csr, sigma: int = 16, keep_f32: bool = False, xcd_slices: bool = False) -> "Csr5":
+    def from_csr(cls, csr, sigma: int = 16, keep_f32: bool = False, xcd_slices: bool = False,
+                 plain_columns: bool = False) -> "Csr5":
         """csr5::from_csr (csr5.h:791-803). `csr` is a Csr or a (rows, cols,
         row_ptr, col_idx, vals) tuple of host arrays. sigma=0 runs the
         autotuner (times the candidate sigmas on this matrix, and the
         column-sliced form for large scattered x). xcd_slices=True forces the
-        column-sliced form for spmv."""
+        column-sliced form for spmv. plain_columns=True leaves out the 4-bit
+        column codes that stencil-like matrices otherwise get."""
         if isinstance(csr, Csr):
             rows, cols = csr.dimensions.rows, csr.dimensions.cols
             rp, ci, v = csr.row_start_offsets, csr.col_indices, csr.values
@@ -159,7 +161,8 @@ class Csr5:
         v = np.ascontiguousarray(v, dtype=np.float64)
         if len(rp) != rows + 1:
             raise ValueError("row_ptr must have rows + 1 entries")
-        flags = (N.DIM_CSR5_KEEP_F32 if keep_f32 else 0) | (N.DIM_CSR5_XCD_SLICES if xcd_slices else 0)
+        flags = ((N.DIM_CSR5_KEEP_F32 if keep_f32 else 0) | (N.DIM_CSR5_XCD_SLICES if xcd_slices else 0) |
+                 (N.DIM_CSR5_PLAIN_COLUMNS if plain_columns else 0))
         opts = N.Csr5Opts(sigma, flags)
         h = C.c_void_p()
         N.check(N.lib().dim_csr5_from_csr(rows, cols, len(v), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,

@@ -43,7 +43,7 @@ void free_handle(dim_csr5* h) {
     delete h;
 }
 
-dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out) {
+dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out, bool plain_columns) {
     std::unique_ptr<dim_csr5, void (*)(dim_csr5*)> h(new dim_csr5, free_handle);
     DIM_HIP_TRY(hipGetDevice(&h->device));
     DevFormat& d = h->f;
@@ -83,6 +83,12 @@ dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out) {
         DIM_HIP_TRY(upload(h.get(), d.vals32, v32));
     }
     DIM_HIP_TRY(upload(h.get(), d.empty_rows, f.empty_rows));
+    if (!f.col_dict.empty() && !plain_columns) {
+        std::vector<int32_t> dict(f.col_dict);
+        dict.resize(kMaxDiagonals, 0);
+        DIM_HIP_TRY(upload(h.get(), d.col_dict, dict));
+        DIM_HIP_TRY(upload(h.get(), d.col_code, f.col_code));
+    }
     for (uint32_t t = 0; t < f.tiles; ++t)
         h->n_dirty += (f.tile_ptr[t] & kDirty) ? 1 : 0;
 
@@ -143,6 +149,7 @@ DevFormat system_view(const dim_csr5* A, uint32_t s) {
     view.col_idx = A->f.col_idx + t0 * tile;
     view.vals64 = A->f.vals64 + t0 * tile;
     view.vals32 = A->f.vals32 ? A->f.vals32 + t0 * tile : nullptr;
+    view.col_code = A->f.col_code ? A->f.col_code + size_t{t0} * kOmega : nullptr;
     view.nnz = A->sys_nnz_csr[s + 1] - A->sys_nnz_csr[s];
     view.padded = size_t{view.tiles} * tile;
     return view;
@@ -275,7 +282,7 @@ hipError_t plain_product(dim_csr5* h, bool f32, const void* x, void* y, hipStrea
 // slots) against registers per lane and load-balance granularity.
 dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,
                           const uint32_t* col_idx, const double* vals, bool keep_f32, bool force_slices,
-                          dim_csr5** out) {
+                          bool plain_columns, dim_csr5** out) {
     const std::vector<uint32_t> candidates = sigma_candidates(rows, nnz);
     dim_csr5* best = nullptr;
     struct Tried {
@@ -330,7 +337,7 @@ dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint
             break;
         }
         dim_csr5* h = nullptr;
-        st = upload_format(f, keep_f32, &h);
+        st = upload_format(f, keep_f32, &h, plain_columns);
         if (st != DIM_SUCCESS)
             break;
         float ms = 0;
@@ -452,16 +459,17 @@ dim_status dim_csr5_from_csr(uint32_t rows, uint32_t cols, uint64_t nnz, const u
     const uint32_t sigma = opts ? opts->sigma : 16;
     const bool keep_f32 = opts && (opts->flags & DIM_CSR5_KEEP_F32);
     const bool slices = opts && (opts->flags & DIM_CSR5_XCD_SLICES);
+    const bool plain_columns = opts && (opts->flags & DIM_CSR5_PLAIN_COLUMNS);
     if (slices && uint64_t{rows} * kXcdSlices >= (1ull << 31))
         return fail(DIM_INVALID_ARGUMENT, "DIM_CSR5_XCD_SLICES needs rows < 2^28");
     if (sigma == 0)
-        return autotune_sigma(rows, cols, nnz, row_ptr, col_idx, vals, keep_f32, slices, out);
+        return autotune_sigma(rows, cols, nnz, row_ptr, col_idx, vals, keep_f32, slices, plain_columns, out);
     HostFormat f;
     err = build_format(rows, cols, nnz, row_ptr, col_idx, vals, sigma, f);
     if (!err.empty())
         return fail(DIM_INVALID_ARGUMENT, err);
     dim_csr5* h = nullptr;
-    if (dim_status st = upload_format(f, keep_f32, &h))
+    if (dim_status st = upload_format(f, keep_f32, &h, plain_columns))
         return st;
     if (slices && nnz) {
         const StackedCsr sc = stack_slices(rows, cols, nnz, row_ptr, col_idx, vals, kXcdSlices);
@@ -498,6 +506,7 @@ dim_status dim_csr5_get_info(const dim_csr5* A, dim_csr5_info* info) {
     info->device_bytes = A->device_bytes + (A->xs ? A->xs->device_bytes : 0);
     info->xcd_slices = A->xs ? A->xs_slices : 0;
     info->xcd_sigma = A->xs ? A->xs->f.sigma : 0;
+    info->column_codes = A->f.col_code != nullptr;
     return DIM_SUCCESS;
 }
 

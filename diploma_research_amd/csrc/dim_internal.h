@@ -55,7 +55,21 @@ struct HostFormat {
     std::vector<double> vals;
     std::vector<uint32_t> empty_rows;
     bool any_dirty = false;
+    // Column codes (see build_col_codes): empty unless every element lies on
+    // one of at most kMaxDiagonals diagonals and no row is empty.
+    std::vector<int32_t> col_dict;  // the diagonals col - row, ascending
+    std::vector<uint32_t> col_code; // T*64 words: nibble r = code of step r
 };
+
+// Column codes: for matrices whose elements lie on at most kMaxDiagonals
+// diagonals (col - row takes few values, as in stencil operators) and have no
+// empty rows, each element's column is also stored as a 4-bit index into the
+// table of those diagonals, packed per lane and tile into one 32-bit word
+// (sigma <= 8). The product then rebuilds col = row + table[code] from the
+// row starts instead of streaming 4-byte column indices.
+constexpr uint32_t kMaxDiagonals = 16;
+constexpr uint32_t kMaxCodedSigma = 8;
+void build_col_codes(HostFormat& f, const uint32_t* row_ptr, const uint32_t* col_idx);
 
 // Builds the format from CSR. Returns an empty string or an error message.
 std::string build_format(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* row_ptr,

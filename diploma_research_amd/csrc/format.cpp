@@ -10,6 +10,8 @@
 #include "dim_internal.h"
 
 #include <algorithm>
+#include <climits>
+#include <mutex>
 #include <thread>
 
 namespace dimgpu {
@@ -186,7 +188,66 @@ std::string build_format(uint32_t rows, uint32_t cols, uint64_t nnz, const uint3
     for (uint64_t r = 0; r < rows; ++r)
         if (row_ptr[r] == row_ptr[r + 1])
             f.empty_rows.push_back(static_cast<uint32_t>(r));
+    build_col_codes(f, row_ptr, col_idx);
     return {};
+}
+
+void build_col_codes(HostFormat& f, const uint32_t* row_ptr, const uint32_t* col_idx) {
+    f.col_dict.clear();
+    f.col_code.clear();
+    if (f.sigma > kMaxCodedSigma || f.any_dirty || !f.nnz || !f.empty_rows.empty())
+        return;
+    // the diagonals, gathered per thread range and merged; more than
+    // kMaxDiagonals (or one beyond 32-bit range) means no codes
+    std::vector<std::vector<int64_t>> parts;
+    std::mutex m;
+    bool too_many = false;
+    parallel_for(f.rows, [&](uint64_t b, uint64_t e) {
+        std::vector<int64_t> d;
+        for (uint64_t r = b; r < e && d.size() <= kMaxDiagonals; ++r)
+            for (uint32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+                const int64_t v = int64_t{col_idx[k]} - static_cast<int64_t>(r);
+                if (std::find(d.begin(), d.end(), v) == d.end()) {
+                    d.push_back(v);
+                    if (d.size() > kMaxDiagonals)
+                        break;
+                }
+            }
+        std::lock_guard<std::mutex> g(m);
+        if (d.size() > kMaxDiagonals)
+            too_many = true;
+        parts.push_back(std::move(d));
+    });
+    if (too_many)
+        return;
+    std::vector<int64_t> all;
+    for (auto& d : parts)
+        all.insert(all.end(), d.begin(), d.end());
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    if (all.size() > kMaxDiagonals || all.front() < INT32_MIN || all.back() > INT32_MAX)
+        return;
+    f.col_dict.assign(all.begin(), all.end());
+    // 4-bit code of every element (padding: code 0), packed per tile column
+    const uint64_t tile = uint64_t{kOmega} * f.sigma;
+    std::vector<uint8_t> code(f.nnz);
+    parallel_for(f.rows, [&](uint64_t b, uint64_t e) {
+        for (uint64_t r = b; r < e; ++r)
+            for (uint32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+                const int64_t v = int64_t{col_idx[k]} - static_cast<int64_t>(r);
+                code[k] = static_cast<uint8_t>(std::lower_bound(all.begin(), all.end(), v) - all.begin());
+            }
+    });
+    f.col_code.assign(uint64_t{f.tiles} * kOmega, 0);
+    parallel_for(uint64_t{f.tiles} * kOmega, [&](uint64_t b, uint64_t e) {
+        for (uint64_t w = b; w < e; ++w) {
+            const uint64_t first = (w / kOmega) * tile + (w % kOmega) * f.sigma;
+            uint32_t word = 0;
+            for (uint32_t r = 0; r < f.sigma && first + r < f.nnz; ++r)
+                word |= uint32_t{code[first + r]} << (4 * r);
+            f.col_code[w] = word;
+        }
+    });
 }
 
 } // namespace dimgpu

@@ -77,7 +77,8 @@ hipError_t upload(dim_csr5* h, T*& p, const std::vector<T>& v) {
 }
 
 // Uploads a host format into a fresh handle (allocates scratch too).
-dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out);
+// plain_columns: leave out the column codes (products stream col_idx).
+dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out, bool plain_columns = false);
 
 // One system's share of a batched matrix as a stand-alone format view.
 DevFormat system_view(const dim_csr5* A, uint32_t s);

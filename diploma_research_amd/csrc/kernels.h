@@ -46,6 +46,9 @@ struct DevFormat {
     uint32_t* empty_rows = nullptr;
     uint32_t n_empty = 0;
     bool any_dirty = false;
+    // column codes (HostFormat::col_code / col_dict), or null
+    uint32_t* col_code = nullptr;
+    int32_t* col_dict = nullptr; // kMaxDiagonals entries
     // host-side tile flags of the whole matrix; tile t of this view is tile
     // binfo_base + t there
     const BoundaryInfo* binfo = nullptr;

@@ -139,6 +139,9 @@ TileParams make_tile_params(const DevFormat& A, bool f32) {
     tp.tdo_ptr = A.tile_desc_offset_ptr;
     tp.tdo = A.tile_desc_offset;
     tp.col_idx = A.col_idx;
+    tp.col_code = A.col_code;
+    tp.col_dict = A.col_dict;
+    tp.cols = A.cols;
     tp.vals = f32 ? static_cast<const void*>(A.vals32) : static_cast<const void*>(A.vals64);
     tp.nnz = A.nnz;
     tp.tile_begin = 0;

@@ -181,6 +181,9 @@ struct TileParams {
     const uint32_t* tdo_ptr;
     const uint32_t* tdo;
     const uint32_t* col_idx;
+    const uint32_t* col_code; // column codes and their diagonals, or null
+    const int32_t* col_dict;
+    uint32_t cols;
     const void* vals;
     const void* x;
     void* y;
@@ -211,7 +214,9 @@ constexpr int smm_group() {
 }
 constexpr int kSmmRegSigma = 16;
 
-template<typename V, int SIGMA, int KMAX, bool DOT>
+// CODES: the matrix carries column codes (every tile clean, sigma <= 8); the
+// kernel then streams one code word per lane instead of sigma column indices.
+template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES = false>
 __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     if (p.done && *p.done)
         return;
@@ -273,14 +278,18 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     constexpr bool XREG = KMAX == 1;
     constexpr bool REG = XREG || SIGMA <= kSmmRegSigma;
     constexpr bool PIPE = XREG && SIGMA <= 8;
+    constexpr bool CODED = CODES && REG && SIGMA <= 8;
     constexpr int NREG = REG ? SIGMA : 1;
     constexpr int NX = XREG ? SIGMA : 1;
     const V* __restrict__ vals_all = static_cast<const V*>(p.vals);
     const W* __restrict__ bits_all = static_cast<const W*>(p.bit_flag);
+    // the diagonals of a coded matrix, lane d holding diagonal d
+    const int dtab = CODED && lane < 16 ? p.col_dict[lane] : 0;
 
     struct Stage {
         V v[NREG];
         uint32_t col[NREG];
+        uint32_t code;
         W bits;
         uint32_t tile_ptr, tdo_at;
         uint32_t tf, tf_next;
@@ -297,8 +306,11 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r) {
                 s.v[r] = ldg_stream<!DOT>(vt + r * 64 + ulane);
-                s.col[r] = ldg_stream<!DOT>(ct + r * 64 + ulane);
+                if constexpr (!CODED)
+                    s.col[r] = ldg_stream<!DOT>(ct + r * 64 + ulane);
             }
+            if constexpr (CODED)
+                s.code = ldg_stream<!DOT>(p.col_code + base / SIGMA + ulane);
         }
         s.bits = (bits_all + base / SIGMA)[ulane];
         // tile_flags and tile_desc_offset_ptr hold T + 1 entries
@@ -306,6 +318,25 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         s.tdo_at = sload(p.tdo_ptr + t);
         s.tf = sload_flag(p.tile_flags + t);
         s.tf_next = sload_flag(p.tile_flags + t + 1);
+    };
+    // Columns of a coded tile: the element at step r of a lane belongs to row
+    // row0 + y_off - 1 + (row starts of the lane at steps <= r), and its
+    // column is that row plus the diagonal its code names. Clamped to the
+    // column range, so that the padding of the last tile (code 0) reads a
+    // valid x entry; padded elements contribute zero regardless.
+    auto decode = [&](Stage& s) {
+        if constexpr (CODED) {
+            const uint32_t bits = static_cast<uint32_t>(s.bits) | (lane == 0 ? 1u : 0u);
+            const int nb = __popc(bits);
+            const int y_off = wave_incl_scan(nb, lane) - nb;
+            const uint32_t rb = (__builtin_amdgcn_readfirstlane(s.tile_ptr) & ~kDirty) + static_cast<uint32_t>(y_off) - 1u;
+#pragma unroll
+            for (int r = 0; r < SIGMA; ++r) {
+                const uint32_t row = rb + static_cast<uint32_t>(__popc(bits & ((2u << r) - 1u)));
+                const int d = __builtin_amdgcn_ds_bpermute(static_cast<int>((s.code >> (4 * r)) & 15u) << 2, dtab);
+                s.col[r] = min(row + static_cast<uint32_t>(d), p.cols - 1u);
+            }
+        }
     };
     Stage nx;
     if constexpr (PIPE)
@@ -320,6 +351,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         V xg[NX];
         if constexpr (PIPE) {
             cu = nx;
+            decode(cu);
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r)
                 xg[r] = x[cu.col[r]];
@@ -331,6 +363,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 fetch(t + 1, nx);
         } else {
             fetch(t, cu);
+            decode(cu);
             if constexpr (XREG) {
 #pragma unroll
                 for (int r = 0; r < SIGMA; ++r)
@@ -857,12 +890,13 @@ struct Plan {
 
 int device_cus();
 
-template<typename V, int SIGMA, int KMAX, bool DOT>
+template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES>
 Plan make_plan(uint32_t ntiles) {
     static int blocks_per_cu = 0;
     if (!blocks_per_cu) {
         int b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_csr5_tiles<V, SIGMA, KMAX, DOT>, 256, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_csr5_tiles<V, SIGMA, KMAX, DOT, CODES>, 256, 0) !=
+              hipSuccess ||
             b <= 0)
             b = 2;
         blocks_per_cu = b;
@@ -878,11 +912,21 @@ Plan make_plan(uint32_t ntiles) {
     return pl;
 }
 
+// Coded kernels exist for single-vector products at sigma <= 8.
+template<int KMAX, int S>
+constexpr bool coded_kernel() {
+    return KMAX == 1 && S <= static_cast<int>(kMaxCodedSigma);
+}
+
 template<typename V, int KMAX, bool DOT>
-Plan plan_by_sigma(uint32_t sigma, uint32_t ntiles) {
+Plan plan_by_sigma(uint32_t sigma, uint32_t ntiles, bool codes) {
     switch (sigma) {
 #define DIM_PLAN_CASE(S)                                                                                              \
-    case S: return make_plan<V, S, KMAX, DOT>(ntiles);
+    case S:                                                                                                           \
+        if constexpr (coded_kernel<KMAX, S>())                                                                        \
+            if (codes)                                                                                                \
+                return make_plan<V, S, KMAX, DOT, true>(ntiles);                                                      \
+        return make_plan<V, S, KMAX, DOT, false>(ntiles);
         DIM_FOR_EACH_SIGMA(DIM_PLAN_CASE)
 #undef DIM_PLAN_CASE
         default: return Plan{};
@@ -890,10 +934,17 @@ Plan plan_by_sigma(uint32_t sigma, uint32_t ntiles) {
 }
 
 template<typename V, int KMAX, bool DOT>
-void launch_tiles_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, hipStream_t st) {
+void launch_tiles_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, hipStream_t st, bool codes) {
     switch (sigma) {
 #define DIM_TILES_CASE(S)                                                                                             \
-    case S: hipLaunchKernelGGL((k_csr5_tiles<V, S, KMAX, DOT>), dim3(grid), dim3(256), 0, st, tp); break;
+    case S:                                                                                                           \
+        if constexpr (coded_kernel<KMAX, S>())                                                                        \
+            if (codes) {                                                                                              \
+                hipLaunchKernelGGL((k_csr5_tiles<V, S, KMAX, DOT, true>), dim3(grid), dim3(256), 0, st, tp);          \
+                break;                                                                                                \
+            }                                                                                                         \
+        hipLaunchKernelGGL((k_csr5_tiles<V, S, KMAX, DOT, false>), dim3(grid), dim3(256), 0, st, tp);                 \
+        break;
         DIM_FOR_EACH_SIGMA(DIM_TILES_CASE)
 #undef DIM_TILES_CASE
         default: break;
@@ -905,7 +956,9 @@ void launch_tiles_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, 
 template<typename V, int KMAX>
 uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, hipStream_t st, hipEvent_t mid) {
     const uint32_t ntiles = tp.tile_end - tp.tile_begin;
-    const Plan pl = dot ? plan_by_sigma<V, KMAX, true>(A.sigma, ntiles) : plan_by_sigma<V, KMAX, false>(A.sigma, ntiles);
+    const bool codes = A.col_code != nullptr;
+    const Plan pl = dot ? plan_by_sigma<V, KMAX, true>(A.sigma, ntiles, codes)
+                        : plan_by_sigma<V, KMAX, false>(A.sigma, ntiles, codes);
     tp.tpw = fp.tpw = pl.tpw;
     tp.nchunks = fp.nchunks = pl.nchunks;
     // Single-vector products without a dot finish rows across chunk
@@ -922,9 +975,9 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
     }
     if (pl.tgrid) {
         if (dot)
-            launch_tiles_by_sigma<V, KMAX, true>(A.sigma, tp, pl.tgrid, st);
+            launch_tiles_by_sigma<V, KMAX, true>(A.sigma, tp, pl.tgrid, st, codes);
         else
-            launch_tiles_by_sigma<V, KMAX, false>(A.sigma, tp, pl.tgrid, st);
+            launch_tiles_by_sigma<V, KMAX, false>(A.sigma, tp, pl.tgrid, st, codes);
     }
     if (mid)
         (void)hipEventRecord(mid, st);

@@ -24,6 +24,7 @@ DIM_F64 = 0
 DIM_F32 = 1
 DIM_CSR5_KEEP_F32 = 1
 DIM_CSR5_XCD_SLICES = 2
+DIM_CSR5_PLAIN_COLUMNS = 4
 DIM_CG_F64 = 0
 DIM_CG_F32_REFINED = 1
 DIM_CG_TRANSPOSE = 1
@@ -46,7 +47,8 @@ class Csr5Info(C.Structure):
     _fields_ = [("rows", C.c_uint32), ("cols", C.c_uint32), ("nnz", C.c_uint64), ("sigma", C.c_uint32),
                 ("omega", C.c_uint32), ("tile_count", C.c_uint32), ("tail_partition_start", C.c_uint32),
                 ("n_empty_rows", C.c_uint32), ("n_dirty_tiles", C.c_uint32), ("has_f32", C.c_uint32),
-                ("device_bytes", C.c_uint64), ("xcd_slices", C.c_uint32), ("xcd_sigma", C.c_uint32)]
+                ("device_bytes", C.c_uint64), ("xcd_slices", C.c_uint32), ("xcd_sigma", C.c_uint32),
+                ("column_codes", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class CgOpts(C.Structure):

@@ -56,6 +56,11 @@ typedef struct dim_csr5_opts {
                                   large x: each XCD gathers from one eighth of x). With sigma = 0 the
                                   autotuner adds it on its own when it is faster, and the flag forces
                                   it. No reference counterpart. */
+#define DIM_CSR5_PLAIN_COLUMNS 4u /* do not keep column codes: products always stream the 32-bit column
+                                     indices (by default a matrix whose elements lie on at most 16
+                                     diagonals, with no empty rows and sigma <= 8, also keeps a 4-bit
+                                     code per element and single-vector products read those). No
+                                     reference counterpart. */
 
 typedef struct dim_csr5_info {
     uint32_t rows, cols;
@@ -67,8 +72,10 @@ typedef struct dim_csr5_info {
     uint32_t n_dirty_tiles;
     uint32_t has_f32;
     uint64_t device_bytes;
-    uint32_t xcd_slices; /* column slices used by dim_csr5_spmv (0: none) */
-    uint32_t xcd_sigma;  /* tile height of the sliced form */
+    uint32_t xcd_slices;   /* column slices used by dim_csr5_spmv (0: none) */
+    uint32_t xcd_sigma;    /* tile height of the sliced form */
+    uint32_t column_codes; /* 1: single-vector products read 4-bit column codes */
+    uint32_t reserved;
 } dim_csr5_info;
 
 const char* dim_last_error(void);

@@ -282,6 +282,59 @@ def test_powerlaw_4m_full_size(cuda):
     assert_spmv_close(y, O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
 
 
+@pytest.mark.parametrize("sigma", [4, 5, 6, 7, 8])
+def test_column_codes_equal_plain_columns(cuda, sigma):
+    """Stencil operators keep 4-bit column codes (their elements lie on 7 or 5
+    diagonals): products read the codes and rebuild each column from its
+    row, with the same summation order as the 32-bit columns, so results
+    are bitwise equal to the plain format's, FP64 and FP32, whole and by
+    tile blocks; with integer inputs they equal the oracle exactly."""
+    import torch
+
+    for rp, ci, v, n in [(*matgen.poisson3d(40), 40 ** 3), (*matgen.poisson2d(300), 300 ** 2)]:
+        A = Csr5.from_csr((n, n, rp, ci, v), sigma=sigma, keep_f32=True)
+        B = Csr5.from_csr((n, n, rp, ci, v), sigma=sigma, keep_f32=True, plain_columns=True)
+        assert A.info.column_codes == 1 and B.info.column_codes == 0
+        xi = np.random.default_rng(sigma).integers(-5, 6, n).astype(np.float64)
+        np.testing.assert_array_equal(A.spmv(to_dev(xi)).cpu().numpy(), O.csr_spmv(rp, ci, v, xi))
+        x = np.random.default_rng(sigma + 1).uniform(-1, 1, n)
+        for dt in (torch.float64, torch.float32):
+            assert torch.equal(A.spmv(to_dev(x, dt)), B.spmv(to_dev(x, dt)))
+        T = A.info.tile_count
+        for tb, te in [(0, T // 3), (T // 3, T), (1, 2)]:
+            fa, ya = A.spmv_block(to_dev(x), tb, te)
+            fb, yb = B.spmv_block(to_dev(x), tb, te)
+            assert fa == fb and torch.equal(ya, yb)
+
+
+def test_column_codes_in_cg_and_when_absent(cuda):
+    """CG's fused product reads the codes too (same iterates bit for bit as
+    the plain format). Matrices with more than 16 diagonals or an empty row
+    keep no codes."""
+    import torch
+
+    from diploma_research_amd import conjugate_gradient
+
+    n = 120 * 120
+    rp, ci, v = matgen.poisson2d(120)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=5)
+    B = Csr5.from_csr((n, n, rp, ci, v), sigma=5, plain_columns=True)
+    assert A.info.column_codes == 1
+    b = to_dev(np.random.default_rng(3).uniform(-1, 1, n))
+    xa, ra = conjugate_gradient(A, b, rtol=1e-10)
+    xb, rb = conjugate_gradient(B, b, rtol=1e-10)
+    assert ra.iterations == rb.iterations and torch.equal(xa, xb)
+    rp2, ci2, v2 = matgen.random_csr(3000, 3000, seed=5, max_len=8)
+    assert Csr5.from_csr((3000, 3000, rp2, ci2, v2), sigma=8).info.column_codes == 0
+    lens = np.diff(rp.astype(np.int64))  # the same operator with row 7 emptied
+    lens[7] = 0
+    keep = np.ones(len(ci), bool)
+    keep[rp[7]:rp[8]] = False
+    rp3 = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    C = Csr5.from_csr((n, n, rp3, ci[keep], v[keep]), sigma=5)
+    assert C.info.column_codes == 0
+
+
 def _slice_case(name):
     if name == "powerlaw":
         n = 200_000
