@@ -283,6 +283,7 @@ def main():
     log("autotuning sigma and converting")
     A0 = Csr5.from_csr((n, n, rp, ci, v), sigma=0)
     sigma = A0.sigma
+    codes = bool(A0.info.column_codes)  # 4-bit column codes (7 diagonals)
     mats = [A0] + [Csr5.from_csr((n, n, rp, ci, v), sigma=sigma) for _ in range(2)]
     x_host = np.random.default_rng(1).uniform(-1, 1, n)
     xs = [torch.from_numpy(x_host).to(dev) for _ in mats]
@@ -357,12 +358,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": "FP64 CSR5 SpMV, 3-D 7-point Poisson 128^3 (n=2097152, nnz=14581760), "
                                "x~U[-1,1]; Infinity Cache defeated by rotating 3 matrix copies",
-                   "sigma": sigma, "omega": 64, "bytes_per_step": bytes_step,
+                   "sigma": sigma, "omega": 64, "bytes_per_step": bytes_step, "column_codes": codes,
                    "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "warm_GB/s": gbs_warm,
                    "parity_rel_err_vs_cpu_baseline": parity},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_csr5_tiles<double,{sigma},1,false>", "kernel_ms": round(kernel_ms, 5),
+                     "kernel": f"k_csr5_tiles<double,{sigma},1,false,{'true' if codes else 'false'}>",
+                     "kernel_ms": round(kernel_ms, 5),
                      "kernels_per_step": 1 if single_kernel else 2,
                      "kernel_ms_source": "HIP events over the timed region, one launch per step" if single_kernel
                      else "HIP events around each tile-kernel launch", "fixup_kernel_ms": round(t_fix, 5)},

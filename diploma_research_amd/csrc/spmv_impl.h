@@ -338,11 +338,21 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             }
         }
     };
-    Stage nx;
+    // Coded single-vector tiles stream 8.5 bytes per element instead of 12;
+    // up to sigma 7 their pipeline runs two tiles ahead (two stage slots, the
+    // chunk walked in pairs so that each slot keeps its registers). Measured
+    // on the 7-point operator: sigma 7 gains 4%, sigma 8 loses 5% to the
+    // lower occupancy.
+    constexpr bool PIPE2 = PIPE && CODED && SIGMA <= 7;
+    constexpr uint32_t AHEAD = PIPE2 ? 2 : 1;
+    Stage nx, nx2;
     if constexpr (PIPE)
         fetch(t0, nx);
+    if constexpr (PIPE2)
+        if (t0 + 1 < t1)
+            fetch(t0 + 1, nx2);
 
-    for (uint32_t t = t0; t < t1; ++t) {
+    auto tile_step = [&](uint32_t t, Stage& slot) {
         const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
         const V* vals = vals_all + base + lane;
         const uint32_t* cols = p.col_idx + base + lane;
@@ -350,17 +360,17 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         Stage cu;
         V xg[NX];
         if constexpr (PIPE) {
-            cu = nx;
+            cu = slot;
             decode(cu);
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r)
                 xg[r] = x[cu.col[r]];
-            // keep the next tile's loads behind this tile's gathers: loads
+            // keep the next tiles' loads behind this tile's gathers: loads
             // return in order, so waiting for the gathers then leaves the next
-            // tile's loads in flight
+            // tiles' loads in flight
             __builtin_amdgcn_sched_barrier(0);
-            if (t + 1 < t1)
-                fetch(t + 1, nx);
+            if (t + AHEAD < t1)
+                fetch(t + AHEAD, slot);
         } else {
             fetch(t, cu);
             decode(cu);
@@ -760,6 +770,16 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 }
             }
         }
+    };
+    if constexpr (PIPE2) {
+        for (uint32_t t = t0; t < t1; t += 2) {
+            tile_step(t, nx);
+            if (t + 1 < t1)
+                tile_step(t + 1, nx2);
+        }
+    } else {
+        for (uint32_t t = t0; t < t1; ++t)
+            tile_step(t, nx);
     }
     if (!chunk_starts && !carry_is_head && !drop_head)
         cflags |= kChunkHead;
@@ -912,10 +932,10 @@ Plan make_plan(uint32_t ntiles) {
     return pl;
 }
 
-// Coded kernels exist for single-vector products at sigma <= 8.
+// Coded kernels exist for sigma <= 8.
 template<int KMAX, int S>
 constexpr bool coded_kernel() {
-    return KMAX == 1 && S <= static_cast<int>(kMaxCodedSigma);
+    return S <= static_cast<int>(kMaxCodedSigma);
 }
 
 template<typename V, int KMAX, bool DOT>
