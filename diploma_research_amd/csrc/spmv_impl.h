@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         s.bits = (bits_all + base / SIGMA)[ulane];
         // tile_flags and tile_desc_offset_ptr hold T + 1 entries
         s.tile_ptr = sload(p.tile_ptr + t);
-        s.tdo_at = sload(p.tdo_ptr + t);
+        s.tdo_at = CODED ? 0u : sload(p.tdo_ptr + t); // coded matrices have no dirty tiles
         s.tf = sload_flag(p.tile_flags + t);
         s.tf_next = sload_flag(p.tile_flags + t + 1);
     };
@@ -348,11 +348,12 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     Stage nx, nx2;
     if constexpr (PIPE)
         fetch(t0, nx);
-    if constexpr (PIPE2)
-        if (t0 + 1 < t1)
-            fetch(t0 + 1, nx2);
 
-    auto tile_step = [&](uint32_t t, Stage& slot) {
+    // pf_tag true: the tile t + AHEAD exists, so the prefetch is issued on
+    // every path (the waits for a slot's loads then have a fixed count of
+    // later loads to leave in flight)
+    auto tile_step = [&](uint32_t t, Stage& slot, auto pf_tag) {
+        constexpr bool IN_RANGE = decltype(pf_tag)::value;
         const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
         const V* vals = vals_all + base + lane;
         const uint32_t* cols = p.col_idx + base + lane;
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             // return in order, so waiting for the gathers then leaves the next
             // tiles' loads in flight
             __builtin_amdgcn_sched_barrier(0);
-            if (t + AHEAD < t1)
+            if (IN_RANGE || t + AHEAD < t1)
                 fetch(t + AHEAD, slot);
         } else {
             fetch(t, cu);
@@ -726,7 +727,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             if (!starts_row && (nseg != 1 || complete))
                 carry_is_head = false;
         };
-        if (dirty)
+        if (!CODED && dirty) // coded matrices have no dirty tiles
             process(std::true_type{});
         else
             process(std::false_type{});
@@ -772,14 +773,25 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         }
     };
     if constexpr (PIPE2) {
-        for (uint32_t t = t0; t < t1; t += 2) {
-            tile_step(t, nx);
+        // the main loop is entered with both slots loading on every path
+        uint32_t t = t0;
+        if (t0 + 3 < t1) {
+            fetch(t0 + 1, nx2);
+            for (; t + 3 < t1; t += 2) {
+                tile_step(t, nx, std::true_type{});
+                tile_step(t + 1, nx2, std::true_type{});
+            }
+        } else if (t0 + 1 < t1) {
+            fetch(t0 + 1, nx2);
+        }
+        for (; t < t1; t += 2) {
+            tile_step(t, nx, std::false_type{});
             if (t + 1 < t1)
-                tile_step(t + 1, nx2);
+                tile_step(t + 1, nx2, std::false_type{});
         }
     } else {
         for (uint32_t t = t0; t < t1; ++t)
-            tile_step(t, nx);
+            tile_step(t, nx, std::false_type{});
     }
     if (!chunk_starts && !carry_is_head && !drop_head)
         cflags |= kChunkHead;
