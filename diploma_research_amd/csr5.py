@@ -149,8 +149,9 @@ class Csr5:
         row_ptr, col_idx, vals) tuple of host arrays. sigma=0 runs the
         autotuner (times the candidate sigmas on this matrix, and the
         column-sliced form for large scattered x). xcd_slices=True forces the
-        column-sliced form for spmv. plain_columns=True leaves out the 4-bit
-        column codes that stencil-like matrices otherwise get."""
+        column-sliced form for spmv. plain_columns=True leaves out the
+        column codes (3 or 4 bits per element) that stencil-like matrices
+        otherwise get."""
         if isinstance(csr, Csr):
             rows, cols = csr.dimensions.rows, csr.dimensions.cols
             rp, ci, v = csr.row_start_offsets, csr.col_indices, csr.values

@@ -56,19 +56,22 @@ struct HostFormat {
     std::vector<uint32_t> empty_rows;
     bool any_dirty = false;
     // Column codes (see build_col_codes): empty unless every element lies on
-    // one of at most kMaxDiagonals diagonals and no row is empty.
+    // one of few diagonals and no row is empty.
     std::vector<int32_t> col_dict;  // the diagonals col - row, ascending
-    std::vector<uint32_t> col_code; // T*64 words: nibble r = code of step r
+    std::vector<uint32_t> col_code; // T*64 words, see build_col_codes
 };
 
-// Column codes: for matrices whose elements lie on at most kMaxDiagonals
-// diagonals (col - row takes few values, as in stencil operators) and have no
-// empty rows, each element's column is also stored as a 4-bit index into the
-// table of those diagonals, packed per lane and tile into one 32-bit word
-// (sigma <= 8). The product then rebuilds col = row + table[code] from the
-// row starts instead of streaming 4-byte column indices.
+// Column codes: for matrices whose elements lie on few diagonals (col - row
+// takes few values, as in stencil operators) and have no empty rows, each
+// lane's column of a tile is also stored as one 32-bit word: bits [0, 24)
+// hold the sigma codes of its elements, code_bits(sigma) bits each, an index
+// into the table of diagonals; bits [24, 24 + sigma) hold its row-start flags
+// (the bit_flag word). The product then rebuilds col = row + table[code] from
+// the row starts instead of streaming 4-byte column indices and the flags.
 constexpr uint32_t kMaxDiagonals = 16;
 constexpr uint32_t kMaxCodedSigma = 8;
+constexpr uint32_t kCodeFlagShift = 24;
+constexpr uint32_t code_bits(uint32_t sigma) { return sigma <= 6 ? 4 : 3; }
 void build_col_codes(HostFormat& f, const uint32_t* row_ptr, const uint32_t* col_idx);
 
 // Builds the format from CSR. Returns an empty string or an error message.

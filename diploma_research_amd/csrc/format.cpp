@@ -225,10 +225,12 @@ void build_col_codes(HostFormat& f, const uint32_t* row_ptr, const uint32_t* col
         all.insert(all.end(), d.begin(), d.end());
     std::sort(all.begin(), all.end());
     all.erase(std::unique(all.begin(), all.end()), all.end());
-    if (all.size() > kMaxDiagonals || all.front() < INT32_MIN || all.back() > INT32_MAX)
+    const uint32_t cb = code_bits(f.sigma);
+    if (all.size() > (1u << cb) || all.front() < INT32_MIN || all.back() > INT32_MAX)
         return;
     f.col_dict.assign(all.begin(), all.end());
-    // 4-bit code of every element (padding: code 0), packed per tile column
+    // code of every element (padding: code 0), packed per tile column with
+    // the column's row-start flags
     const uint64_t tile = uint64_t{kOmega} * f.sigma;
     std::vector<uint8_t> code(f.nnz);
     parallel_for(f.rows, [&](uint64_t b, uint64_t e) {
@@ -242,9 +244,9 @@ void build_col_codes(HostFormat& f, const uint32_t* row_ptr, const uint32_t* col
     parallel_for(uint64_t{f.tiles} * kOmega, [&](uint64_t b, uint64_t e) {
         for (uint64_t w = b; w < e; ++w) {
             const uint64_t first = (w / kOmega) * tile + (w % kOmega) * f.sigma;
-            uint32_t word = 0;
+            uint32_t word = f.bit_flag[w] << kCodeFlagShift;
             for (uint32_t r = 0; r < f.sigma && first + r < f.nnz; ++r)
-                word |= uint32_t{code[first + r]} << (4 * r);
+                word |= uint32_t{code[first + r]} << (cb * r);
             f.col_code[w] = word;
         }
     });

@@ -312,20 +312,24 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             if constexpr (CODED)
                 s.code = ldg_stream<!DOT>(p.col_code + base / SIGMA + ulane);
         }
-        s.bits = (bits_all + base / SIGMA)[ulane];
+        if constexpr (!CODED) // a code word carries its lane's flags
+            s.bits = (bits_all + base / SIGMA)[ulane];
         // tile_flags and tile_desc_offset_ptr hold T + 1 entries
         s.tile_ptr = sload(p.tile_ptr + t);
         s.tdo_at = CODED ? 0u : sload(p.tdo_ptr + t); // coded matrices have no dirty tiles
         s.tf = sload_flag(p.tile_flags + t);
         s.tf_next = sload_flag(p.tile_flags + t + 1);
     };
-    // Columns of a coded tile: the element at step r of a lane belongs to row
+    // Columns of a coded tile (the code word also carries the lane's row-start
+    // flags): the element at step r of a lane belongs to row
     // row0 + y_off - 1 + (row starts of the lane at steps <= r), and its
     // column is that row plus the diagonal its code names. Clamped to the
     // column range, so that the padding of the last tile (code 0) reads a
     // valid x entry; padded elements contribute zero regardless.
     auto decode = [&](Stage& s) {
         if constexpr (CODED) {
+            constexpr uint32_t CB = code_bits(SIGMA);
+            s.bits = static_cast<W>(s.code >> kCodeFlagShift);
             const uint32_t bits = static_cast<uint32_t>(s.bits) | (lane == 0 ? 1u : 0u);
             const int nb = __popc(bits);
             const int y_off = wave_incl_scan(nb, lane) - nb;
@@ -333,7 +337,8 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r) {
                 const uint32_t row = rb + static_cast<uint32_t>(__popc(bits & ((2u << r) - 1u)));
-                const int d = __builtin_amdgcn_ds_bpermute(static_cast<int>((s.code >> (4 * r)) & 15u) << 2, dtab);
+                const int d =
+                  __builtin_amdgcn_ds_bpermute(static_cast<int>((s.code >> (CB * r)) & ((1u << CB) - 1u)) << 2, dtab);
                 s.col[r] = min(row + static_cast<uint32_t>(d), p.cols - 1u);
             }
         }

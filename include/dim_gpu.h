@@ -57,10 +57,11 @@ typedef struct dim_csr5_opts {
                                   autotuner adds it on its own when it is faster, and the flag forces
                                   it. No reference counterpart. */
 #define DIM_CSR5_PLAIN_COLUMNS 4u /* do not keep column codes: products always stream the 32-bit column
-                                     indices (by default a matrix whose elements lie on at most 16
-                                     diagonals, with no empty rows and sigma <= 8, also keeps a 4-bit
-                                     code per element and single-vector products read those). No
-                                     reference counterpart. */
+                                     indices. By default a matrix with no empty rows whose elements
+                                     lie on few diagonals (at most 16 for sigma <= 6, 8 for sigma 7
+                                     and 8) also keeps a 3- or 4-bit code per element, packed with
+                                     the row-start flags into one word per lane and tile, and the
+                                     products read those. No reference counterpart. */
 
 typedef struct dim_csr5_info {
     uint32_t rows, cols;
@@ -74,7 +75,7 @@ typedef struct dim_csr5_info {
     uint64_t device_bytes;
     uint32_t xcd_slices;   /* column slices used by dim_csr5_spmv (0: none) */
     uint32_t xcd_sigma;    /* tile height of the sliced form */
-    uint32_t column_codes; /* 1: single-vector products read 4-bit column codes */
+    uint32_t column_codes; /* 1: products read column codes instead of 32-bit indices */
     uint32_t reserved;
 } dim_csr5_info;
 

@@ -284,7 +284,7 @@ def test_powerlaw_4m_full_size(cuda):
 
 @pytest.mark.parametrize("sigma", [4, 5, 6, 7, 8])
 def test_column_codes_equal_plain_columns(cuda, sigma):
-    """Stencil operators keep 4-bit column codes (their elements lie on 7 or 5
+    """Stencil operators keep column codes (their elements lie on 7 or 5
     diagonals): products read the codes and rebuild each column from its
     row, with the same summation order as the 32-bit columns, so results
     are bitwise equal to the plain format's, FP64 and FP32, whole and by
