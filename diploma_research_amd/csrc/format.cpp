@@ -195,7 +195,9 @@ std::string build_format(uint32_t rows, uint32_t cols, uint64_t nnz, const uint3
 void build_col_codes(HostFormat& f, const uint32_t* row_ptr, const uint32_t* col_idx) {
     f.col_dict.clear();
     f.col_code.clear();
-    if (f.sigma > kMaxCodedSigma || f.any_dirty || !f.nnz || !f.empty_rows.empty())
+    // (products of coded matrices address x with 32-bit byte offsets)
+    if (f.sigma > kMaxCodedSigma || f.any_dirty || !f.nnz || !f.empty_rows.empty() ||
+        uint64_t{f.cols} * sizeof(double) > (uint64_t{1} << 32))
         return;
     // the diagonals, gathered per thread range and merged; more than
     // kMaxDiagonals (or one beyond 32-bit range) means no codes

@@ -299,7 +299,8 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     // compiler from turning the bases into per-lane 64-bit pointers).
     const uint32_t ulane = static_cast<uint32_t>(lane);
     auto fetch = [&](uint32_t t, Stage& s) {
-        const size_t base = static_cast<size_t>(__builtin_amdgcn_readfirstlane(t)) * (64 * SIGMA);
+        const size_t tile64 = static_cast<size_t>(__builtin_amdgcn_readfirstlane(t)) * 64;
+        const size_t base = tile64 * SIGMA;
         if constexpr (REG) {
             const V* vt = vals_all + base;
             const uint32_t* ct = p.col_idx + base;
@@ -310,10 +311,10 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                     s.col[r] = ldg_stream<!DOT>(ct + r * 64 + ulane);
             }
             if constexpr (CODED)
-                s.code = ldg_stream<!DOT>(p.col_code + base / SIGMA + ulane);
+                s.code = ldg_stream<!DOT>(p.col_code + tile64 + ulane);
         }
         if constexpr (!CODED) // a code word carries its lane's flags
-            s.bits = (bits_all + base / SIGMA)[ulane];
+            s.bits = (bits_all + tile64)[ulane];
         // tile_flags and tile_desc_offset_ptr hold T + 1 entries
         s.tile_ptr = sload(p.tile_ptr + t);
         s.tdo_at = CODED ? 0u : sload(p.tdo_ptr + t); // coded matrices have no dirty tiles
@@ -334,12 +335,16 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             const int nb = __popc(bits);
             const int y_off = wave_incl_scan(nb, lane) - nb;
             const uint32_t rb = (__builtin_amdgcn_readfirstlane(s.tile_ptr) & ~kDirty) + static_cast<uint32_t>(y_off) - 1u;
+            // all table lookups in flight together, then the adds
+            int d[SIGMA];
+#pragma unroll
+            for (int r = 0; r < SIGMA; ++r)
+                d[r] = __builtin_amdgcn_ds_bpermute(static_cast<int>((s.code >> (CB * r)) & ((1u << CB) - 1u)) << 2,
+                                                    dtab);
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r) {
                 const uint32_t row = rb + static_cast<uint32_t>(__popc(bits & ((2u << r) - 1u)));
-                const int d =
-                  __builtin_amdgcn_ds_bpermute(static_cast<int>((s.code >> (CB * r)) & ((1u << CB) - 1u)) << 2, dtab);
-                s.col[r] = min(row + static_cast<uint32_t>(d), p.cols - 1u);
+                s.col[r] = min(row + static_cast<uint32_t>(d[r]), p.cols - 1u);
             }
         }
     };
@@ -369,8 +374,13 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             cu = slot;
             decode(cu);
 #pragma unroll
-            for (int r = 0; r < SIGMA; ++r)
-                xg[r] = x[cu.col[r]];
+            for (int r = 0; r < SIGMA; ++r) {
+                if constexpr (CODED) // x of a coded matrix spans less than 4 GB: scalar base + 32-bit offset
+                    xg[r] = *reinterpret_cast<const V*>(reinterpret_cast<const char*>(x) +
+                                                        cu.col[r] * static_cast<uint32_t>(sizeof(V)));
+                else
+                    xg[r] = x[cu.col[r]];
+            }
             // keep the next tiles' loads behind this tile's gathers: loads
             // return in order, so waiting for the gathers then leaves the next
             // tiles' loads in flight
