@@ -273,6 +273,41 @@ def ref_csr_from_coo(rows, cols, ri, ci, v, symmetric=False):
 
 _ref_h5 = None
 
+# ---- The original CSR5 that the reference vendors (third-party/csr5_orig:
+# Liu & Vinter's AVX2 anonymouslib, omega = 4), built by oracle/Makefile into
+# oracle/_ref/libcsr5orig.so: a second CPU oracle for cross-checks.
+
+_orig = None
+
+
+def orig_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libcsr5orig.so"))
+
+
+def orig_spmv(rows, cols, row_ptr, col_idx, vals, x, sigma=16):
+    """y = A x by the original CSR5 (y zero-filled first: it never writes
+    empty rows). Its kernel has sigma = 16 compiled in
+    (ANONYMOUSLIB_CSR5_SIGMA, detail/avx2/common_avx2.h:13): other tile
+    heights are refused."""
+    if sigma != 16:
+        raise ValueError("the original CSR5 kernel is compiled for sigma = 16")
+    global _orig
+    if _orig is None:
+        L = C.CDLL(os.path.join(_HERE, "_ref", "libcsr5orig.so"))
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.csr5orig_spmv.argtypes = [C.c_int, C.c_int, C.c_int, i32p, i32p, _f64p, C.c_int, _f64p, _f64p]
+        L.csr5orig_spmv.restype = C.c_int
+        _orig = L
+    if rows >= 2**31 or cols >= 2**31 or len(vals) >= 2**31:
+        raise ValueError("the original CSR5 takes 32-bit signed sizes")
+    y = np.empty(rows)
+    st = _orig.csr5orig_spmv(rows, cols, len(vals), np.ascontiguousarray(row_ptr, np.int32),
+                             np.ascontiguousarray(col_idx, np.int32), np.ascontiguousarray(vals, np.float64),
+                             int(sigma), np.ascontiguousarray(x, np.float64), y)
+    if st != 0:
+        raise RuntimeError(f"original CSR5 returned status {st}")
+    return y
+
 
 def ref_h5_available() -> bool:
     return os.path.exists(os.path.join(_HERE, "_ref", "libdimref_h5.so"))

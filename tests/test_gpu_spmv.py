@@ -81,6 +81,17 @@ def test_format_matches_oracle_conversion(cuda, sigma):
     np.testing.assert_array_equal(f["empty_rows"], np.nonzero(np.diff(rp.astype(np.int64)) == 0)[0])
 
 
+@pytest.mark.skipif(not O.orig_available(), reason="original CSR5 build absent")
+@pytest.mark.parametrize("sigma", [7, 16, 32])
+def test_gpu_matches_original_csr5(cuda, sigma):
+    """Against the second CPU oracle, the original CSR5 that the reference
+    vendors (omega = 4, sigma = 16): normwise within the parity gate."""
+    rp, ci, v = matgen.random_csr(6000, 5000, seed=sigma, max_len=30, empty_frac=0.1, long_rows=[(3, 4000)])
+    x = np.random.default_rng(sigma).uniform(-1, 1, 5000)
+    A = Csr5.from_csr((6000, 5000, rp, ci, v), sigma=sigma)
+    assert_spmv_close(A.spmv(to_dev(x)).cpu().numpy(), O.orig_spmv(6000, 5000, rp, ci, v, x), rp, ci, v, x)
+
+
 def test_overwrites_y_and_is_deterministic(cuda):
     import torch
 

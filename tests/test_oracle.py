@@ -168,6 +168,50 @@ def test_reference_stale_state_defect_documented():
     np.testing.assert_allclose(A.spmv(x, 4), exact, rtol=0, atol=1e-13)
 
 
+# --------------------------------- the original CSR5 (second oracle) ---
+
+@pytest.mark.skipif(not O.orig_available(), reason="original CSR5 build absent")
+@pytest.mark.parametrize("case", range(5))
+def test_original_csr5_integer_exact(case):
+    """The original CSR5 the reference vendors (anonymouslib, omega = 4, its
+    compiled-in sigma = 16) agrees exactly with the plain CSR product and
+    with the oracle's restatement of the reference on integer inputs (empty
+    rows, long rows, a tail shorter than a tile, tiny matrices)."""
+    rows, cols, ml, ef, lr = [(2000, 1500, 12, 0.1, [(7, 1400), (8, 300)]), (333, 777, 3, 0.5, []), (5, 5, 2, 0.0, []),
+                              (1, 300, 0, 0.0, [(0, 300)]), (4096, 64, 40, 0.02, [(0, 64), (4095, 64)])][case]
+    rp, ci, v = matgen.random_csr(rows, cols, seed=50 + case, max_len=ml, empty_frac=ef, long_rows=lr, integer=True)
+    x = np.random.default_rng(case).integers(-5, 6, cols).astype(np.float64)
+    exact = O.csr_spmv(rp, ci, v, x)
+    np.testing.assert_array_equal(O.orig_spmv(rows, cols, rp, ci, v, x), exact)
+    np.testing.assert_array_equal(O.Csr5(rows, cols, rp, ci, v, 16, 4).spmv(x, 4), exact)
+    with pytest.raises(ValueError, match="sigma = 16"):
+        O.orig_spmv(rows, cols, rp, ci, v, x, 8)
+
+
+@pytest.mark.skipif(not O.orig_available(), reason="original CSR5 build absent")
+@pytest.mark.parametrize("seed", range(4))
+def test_original_csr5_agrees_with_oracle_random(seed):
+    """Floating-point inputs: the original and the oracle sum each row in
+    their own orders; both stay within the componentwise rounding bound
+    (len_i + 2) * u * sum |a x| of the exact product, and within 1e-13
+    normwise of each other."""
+    rng = np.random.default_rng(seed)
+    rows, cols = int(rng.integers(100, 4000)), int(rng.integers(100, 4000))
+    rp, ci, v = matgen.random_csr(rows, cols, seed, max_len=int(rng.integers(1, 40)),
+                                  empty_frac=float(rng.choice([0, 0.05, 0.4])),
+                                  long_rows=[(int(rng.integers(rows)), int(rng.integers(64, min(cols, 3000))))])
+    x = rng.uniform(-1, 1, cols)
+    yo = O.orig_spmv(rows, cols, rp, ci, v, x, 16)
+    ya = O.Csr5(rows, cols, rp, ci, v).spmv(x, 4)
+    lens = np.diff(rp.astype(np.int64))
+    absum = np.bincount(np.repeat(np.arange(rows), lens), weights=np.abs(v * x[ci]), minlength=rows)
+    exact = O.csr_spmv(rp, ci, v, x)
+    u = np.finfo(np.float64).eps / 2
+    for y in (yo, ya):
+        assert (np.abs(y - exact) <= (lens + 2) * u * absum + 1e-300).all()
+    assert np.linalg.norm(yo - ya) <= 1e-13 * np.linalg.norm(exact)
+
+
 # ------------------------------------------------ brute force / closed ---
 
 @pytest.mark.parametrize("sigma,omega", [(16, 4), (8, 4), (16, 64), (32, 64), (4, 8)])
