@@ -163,6 +163,10 @@ dim_status check_handle(const dim_csr5* A) {
     return DIM_SUCCESS;
 }
 
+// row-major multi-RHS blocks with at least this many sides go through
+// column-major staging
+constexpr uint32_t kRowMajorStagingK = 6;
+
 // ---- Column slices ---------------------------------------------------------
 // A product that gathers x entries from all over a large x (random columns)
 // is bound by the rate of L2 misses, not by HBM: every XCD's L2 (4 MB) holds a
@@ -674,8 +678,38 @@ dim_status dim_csr5_spmm(const dim_csr5* A, dim_dtype dtype, uint32_t k, const v
     if ((A->f.cols && !X) || (A->f.rows && !Y))
         return fail(DIM_INVALID_ARGUMENT, "null matrix argument");
     auto* self = const_cast<dim_csr5*>(A);
-    DIM_HIP_TRY(launch_spmv(self->f, self->s, dtype == DIM_F32, static_cast<int>(k), X, Y, 0, A->f.tiles, 0, 0,
-                            A->f.n_empty, false, nullptr, static_cast<hipStream_t>(stream)));
+    const auto st = static_cast<hipStream_t>(stream);
+    const bool f32 = dtype == DIM_F32;
+    if (k >= kRowMajorStagingK && A->f.rows && A->f.cols) {
+        // From 6 sides on, the column-major product plus two transposes beats
+        // gathering k-wide rows (one cache line per lane and side).
+        const size_t esz = f32 ? 4 : 8;
+        const size_t xb = size_t{A->f.cols} * k * esz, need = xb + size_t{A->f.rows} * k * esz;
+        if (self->mm_bytes < need) {
+            if (self->mm_scratch) {
+                auto it = std::find(self->allocations.begin(), self->allocations.end(), self->mm_scratch);
+                if (it != self->allocations.end())
+                    self->allocations.erase(it);
+                (void)hipFree(self->mm_scratch);
+                self->device_bytes -= self->mm_bytes;
+                self->mm_scratch = nullptr;
+                self->mm_bytes = 0;
+            }
+            char* buf = nullptr;
+            DIM_HIP_TRY(dev_alloc(self, buf, need));
+            self->mm_scratch = buf;
+            self->mm_bytes = need;
+        }
+        char* xt = static_cast<char*>(self->mm_scratch);
+        char* yt = xt + xb;
+        DIM_HIP_TRY(launch_block_transpose(f32, true, A->f.cols, static_cast<int>(k), X, xt, st));
+        DIM_HIP_TRY(launch_spmv(self->f, self->s, f32, static_cast<int>(k), xt, yt, 0, A->f.tiles, 0, 0, A->f.n_empty,
+                                false, nullptr, st, nullptr, A->f.cols, A->f.rows));
+        DIM_HIP_TRY(launch_block_transpose(f32, false, A->f.rows, static_cast<int>(k), yt, Y, st));
+        return DIM_SUCCESS;
+    }
+    DIM_HIP_TRY(launch_spmv(self->f, self->s, f32, static_cast<int>(k), X, Y, 0, A->f.tiles, 0, 0, A->f.n_empty, false,
+                            nullptr, st));
     return DIM_SUCCESS;
 }
 

@@ -36,6 +36,10 @@ struct dim_csr5 {
     void* ystack = nullptr;
     uint32_t* present = nullptr;
     uint32_t* present_rank = nullptr;
+    // column-major staging of row-major multi-RHS blocks (dim_csr5_spmm),
+    // grown on demand
+    void* mm_scratch = nullptr;
+    size_t mm_bytes = 0;
 };
 
 namespace dimgpu {

@@ -98,6 +98,10 @@ hipError_t launch_spmv_transpose(const DevFormat& A, bool f32, const void* x, vo
 hipError_t launch_slice_sum(bool f32, const void* ystack, const uint32_t* present, const uint32_t* rank,
                             uint32_t rows, uint32_t slices, void* y, hipStream_t stream);
 
+// n x k blocks from row-major to column-major (to_cols) or back.
+hipError_t launch_block_transpose(bool f32, bool to_cols, size_t n, int k, const void* src, void* dst,
+                                  hipStream_t stream);
+
 // Scatters CSR-order values into the tile layout (FP64, and FP32 if kept).
 hipError_t launch_set_values(const DevFormat& A, const double* vals_csr, hipStream_t stream);
 

@@ -337,6 +337,67 @@ __global__ __launch_bounds__(256) void k_slice_sum(const V* __restrict__ ys, con
 
 } // namespace
 
+namespace {
+
+// n x k blocks between row-major (i*k + j) and column-major (j*n + i),
+// k <= 16: a block moves 256 rows through LDS, so that both the row-major
+// and the column-major side are read and written in contiguous runs.
+constexpr int kTransposeRows = 256;
+template<typename V, bool TO_COLS>
+__global__ __launch_bounds__(256) void k_block_transpose(size_t n, int k, const V* __restrict__ src,
+                                                         V* __restrict__ dst) {
+    __shared__ V tile[16 * (kTransposeRows + 1)];
+    const size_t row0 = static_cast<size_t>(blockIdx.x) * kTransposeRows;
+    const int rows = static_cast<int>(n - row0 < kTransposeRows ? n - row0 : kTransposeRows);
+    const int count = rows * k;
+    if constexpr (TO_COLS) {
+        const V* s = src + row0 * k;
+        for (int e = threadIdx.x; e < count; e += 256)
+            tile[(e % k) * (kTransposeRows + 1) + e / k] = s[e];
+        __syncthreads();
+        for (int e = threadIdx.x; e < count; e += 256) {
+            const int j = e / rows, r = e - j * rows;
+            dst[static_cast<size_t>(j) * n + row0 + r] = tile[j * (kTransposeRows + 1) + r];
+        }
+    } else {
+        for (int e = threadIdx.x; e < count; e += 256) {
+            const int j = e / rows, r = e - j * rows;
+            tile[j * (kTransposeRows + 1) + r] = src[static_cast<size_t>(j) * n + row0 + r];
+        }
+        __syncthreads();
+        V* d = dst + row0 * k;
+        for (int e = threadIdx.x; e < count; e += 256)
+            d[e] = tile[(e % k) * (kTransposeRows + 1) + e / k];
+    }
+}
+
+} // namespace
+
+hipError_t launch_block_transpose(bool f32, bool to_cols, size_t n, int k, const void* src, void* dst,
+                                  hipStream_t stream) {
+    if (!n)
+        return hipSuccess;
+    if (k < 1 || k > 16)
+        return hipErrorInvalidValue;
+    const uint32_t grid = static_cast<uint32_t>((n + kTransposeRows - 1) / kTransposeRows);
+    if (f32) {
+        if (to_cols)
+            hipLaunchKernelGGL((k_block_transpose<float, true>), dim3(grid), dim3(256), 0, stream, n, k,
+                               static_cast<const float*>(src), static_cast<float*>(dst));
+        else
+            hipLaunchKernelGGL((k_block_transpose<float, false>), dim3(grid), dim3(256), 0, stream, n, k,
+                               static_cast<const float*>(src), static_cast<float*>(dst));
+    } else {
+        if (to_cols)
+            hipLaunchKernelGGL((k_block_transpose<double, true>), dim3(grid), dim3(256), 0, stream, n, k,
+                               static_cast<const double*>(src), static_cast<double*>(dst));
+        else
+            hipLaunchKernelGGL((k_block_transpose<double, false>), dim3(grid), dim3(256), 0, stream, n, k,
+                               static_cast<const double*>(src), static_cast<double*>(dst));
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_slice_sum(bool f32, const void* ystack, const uint32_t* present, const uint32_t* rank,
                             uint32_t rows, uint32_t slices, void* y, hipStream_t stream) {
     if (!rows)

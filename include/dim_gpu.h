@@ -137,7 +137,9 @@ dim_status dim_csr5_spmv_block(const dim_csr5* A, dim_dtype dtype, uint32_t tile
 
 /* Multi-RHS product Y = A*X with k in 1..16 right-hand sides decoded from
  * one pass over the tiles. X is cols x k and Y rows x k, row-major (entry
- * (i, j) at i*k + j). No reference counterpart (the reference has only
+ * (i, j) at i*k + j). For k >= 6 the blocks are staged column-major in a
+ * scratch buffer that the matrix keeps (allocated on first use, grown when a
+ * larger block comes). No reference counterpart (the reference has only
  * single-vector spmv). */
 dim_status dim_csr5_spmm(const dim_csr5* A, dim_dtype dtype, uint32_t k, const void* X, void* Y, void* stream);
 
