@@ -318,6 +318,33 @@ def test_column_codes_equal_plain_columns(cuda, sigma):
             assert fa == fb and torch.equal(ya, yb)
 
 
+@pytest.mark.parametrize("shape", ["wide", "tiny", "one_row"])
+def test_column_codes_rectangular_and_small(cuda, shape):
+    """Codes on rectangular banded matrices (diagonals far off the main one)
+    and on matrices smaller than one tile equal the plain columns bit for
+    bit."""
+    import torch
+
+    if shape == "wide":  # row i holds columns i, i + 1000, i + 2000 (3 diagonals)
+        rows, cols, offs = 1000, 3000, [0, 1000, 2000]
+    elif shape == "tiny":
+        rows, cols, offs = 37, 40, [0, 3]
+    else:
+        rows, cols, offs = 1, 500, [0, 7, 499]
+    rp = np.arange(rows + 1, dtype=np.uint32) * len(offs)
+    ci = (np.arange(rows)[:, None] + np.asarray(offs)[None, :]).ravel().astype(np.uint32)
+    v = np.random.default_rng(rows).standard_normal(len(ci))
+    x = np.random.default_rng(cols).uniform(-1, 1, cols)
+    for sigma in (4, 6, 7, 8):
+        A = Csr5.from_csr((rows, cols, rp, ci, v), sigma=sigma, keep_f32=True)
+        B = Csr5.from_csr((rows, cols, rp, ci, v), sigma=sigma, keep_f32=True, plain_columns=True)
+        assert A.info.column_codes == 1
+        ya = A.spmv(to_dev(x))
+        assert torch.equal(ya, B.spmv(to_dev(x)))
+        assert torch.equal(A.spmv(to_dev(x, torch.float32)), B.spmv(to_dev(x, torch.float32)))
+        assert_spmv_close(ya.cpu().numpy(), O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
+
+
 def test_column_codes_in_cg_and_when_absent(cuda):
     """CG's fused product reads the codes too (same iterates bit for bit as
     the plain format). Matrices with more than 16 diagonals or an empty row
