@@ -100,16 +100,25 @@ __device__ void block_store_sum(double acc, double* parts) {
     __syncthreads();
 }
 
-// sum_i parts[i*k + j], i < n, by wave 0 in a fixed order, broadcast through
-// `out` (LDS): every block obtains the same bits.
+// sum_i parts[i*k + j], i < n, in a fixed order (thread t sums i = t,
+// t + blockDim.x, ...; then waves in order), broadcast through `out` (LDS):
+// every block obtains the same bits. All threads load, so a few thousand
+// partials cost one memory round trip.
 __device__ void reduce_col(const double* parts, int n, int k, int j, double* out) {
-    if (threadIdx.x < 64) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < n; i += 64)
-            s += parts[static_cast<size_t>(i) * k + j];
-        s = wave_sum(s);
-        if (threadIdx.x == 0)
-            *out = s;
+    __shared__ double red[kVecThreads / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        s += parts[static_cast<size_t>(i) * k + j];
+    s = wave_sum(s);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0)
+        red[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < nw; ++i)
+            t += red[i];
+        *out = t;
     }
     __syncthreads();
 }
@@ -401,10 +410,12 @@ struct Launcher {
                           double* rr_b, uint32_t nvb, double* hist, int hist_cap, hipStream_t s) {
         constexpr bool f32 = std::is_same<V, float>::value;
         uint32_t n_pq = 1;
-        (void)launch_spmv(A->f, A->s, f32, k, p, q, 0, A->f.tiles, 0, 0, A->f.n_empty, true, &st->done, s, &n_pq, n);
+        const double* pq_parts = A->s.dot_blk;
+        (void)launch_spmv(A->f, A->s, f32, k, p, q, 0, A->f.tiles, 0, 0, A->f.n_empty, true, &st->done, s, &n_pq, n,
+                          0, &pq_parts);
         const dim3 grid(nvb, static_cast<uint32_t>(k));
         hipLaunchKernelGGL((k_cg_update_r<V>), grid, dim3(kVecThreads), 0, s, st, n, r, static_cast<const V*>(q),
-                           A->s.dot_blk, static_cast<int>(n_pq), rr_a, rr_b, static_cast<int>(nvb));
+                           pq_parts, static_cast<int>(n_pq), rr_a, rr_b, static_cast<int>(nvb));
         hipLaunchKernelGGL((k_cg_update_xp<V, KMAX>), grid, dim3(kVecThreads), 0, s, st, n, x,
                            static_cast<const V*>(r), p, rr_b, rr_a, static_cast<int>(nvb), hist, hist_cap);
     }

@@ -70,8 +70,11 @@ struct Scratch {
 // y = A x over tiles [tile_begin, tile_end). x and y are n x k, row-major.
 // row_base: y row index offset (0 for the whole matrix; first row of a block).
 // When `dot` is set (square matrices, whole matrix only) the per-column
-// products x[row]*y[row] are reduced into scratch.dot_blk, one partial per
-// fix-up block; *dot_blocks (if given) receives that count.
+// products x[row]*y[row] are left as partials, column j of partial i at
+// parts[i*k + j]: *dot_parts (if given) receives parts (scratch.dot_blk, one
+// partial per fix-up block, or scratch.dot_chunk, one per chunk, when a
+// single-vector product needs no fix-up kernel) and *dot_blocks (if given)
+// the number of partials.
 // `done` (may be null): a device flag; when *done != 0 the kernels return at
 // once (used by solver launch sequences).
 // ldx: 0 for row-major n x k blocks (element (i, j) at i*k + j); otherwise the
@@ -80,7 +83,8 @@ struct Scratch {
 hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y,
                        uint32_t tile_begin, uint32_t tile_end, uint32_t row_base, uint32_t empty_begin,
                        uint32_t empty_end, bool dot, const int* done, hipStream_t stream,
-                       uint32_t* dot_blocks = nullptr, uint64_t ldx = 0, uint64_t ldy = 0);
+                       uint32_t* dot_blocks = nullptr, uint64_t ldx = 0, uint64_t ldy = 0,
+                       const double** dot_parts = nullptr);
 
 // Same as launch_spmv over the whole matrix with events recorded before the
 // tile kernel, between the two kernels and after the fix-up kernel.

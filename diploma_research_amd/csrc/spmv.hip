@@ -245,14 +245,16 @@ void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x,
 
 hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
                        uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, bool dot,
-                       const int* done, hipStream_t stream, uint32_t* dot_blocks, uint64_t ldx, uint64_t ldy) {
+                       const int* done, hipStream_t stream, uint32_t* dot_blocks, uint64_t ldx, uint64_t ldy,
+                       const double** dot_parts) {
     TileParams tp;
     FixParams fp;
     fill_params(A, S, f32, k, x, y, tile_begin, tile_end, row_base, empty_begin, empty_end, done, tp, fp, ldx, ldy);
     uint32_t nb = 0;
+    const double* parts = S.dot_blk;
     if (k == 1) {
-        nb = f32 ? launch_pair<float, 1>(A, tp, fp, dot, stream, nullptr)
-                 : launch_pair<double, 1>(A, tp, fp, dot, stream, nullptr);
+        nb = f32 ? launch_pair<float, 1>(A, tp, fp, dot, stream, nullptr, &parts)
+                 : launch_pair<double, 1>(A, tp, fp, dot, stream, nullptr, &parts);
     } else {
         // Wide blocks may run as column groups over the same tiles (each group
         // reads the matrix again). Every group runs the same kernel width, so
@@ -281,6 +283,8 @@ hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const vo
     }
     if (dot_blocks)
         *dot_blocks = nb;
+    if (dot_parts)
+        *dot_parts = parts;
     return hipGetLastError();
 }
 

@@ -253,13 +253,13 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     uint8_t cflags = chunk_starts ? kChunkStarts : 0;
     bool row_start_seen = false;
 
-    // Look-ahead (single-vector products without a fused dot): a row that
-    // starts in the last tile of a chunk and ends in the first tile of the
-    // next chunk is finished by the earlier chunk, which reads the few
-    // elements of that row in the next tile; the later chunk drops its part.
-    // Other rows that cross chunks go through the head/tail slots and the
-    // fix-up kernel. Both chunks decide from the same two tile flags.
-    constexpr bool LOOK = KMAX == 1 && !DOT;
+    // Look-ahead (single-vector products): a row that starts in the last tile
+    // of a chunk and ends in the first tile of the next chunk is finished by
+    // the earlier chunk, which reads the few elements of that row in the next
+    // tile; the later chunk drops its part. Other rows that cross chunks go
+    // through the head/tail slots and the fix-up kernel. Both chunks decide
+    // from the same two tile flags.
+    constexpr bool LOOK = KMAX == 1;
     const bool look = LOOK && p.lookahead;
     bool drop_head = false;
     if (look && !chunk_starts && t0 > p.tile_begin && (tf0 & kMultiSeg))
@@ -362,6 +362,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     // pf_tag true: the tile t + AHEAD exists, so the prefetch is issued on
     // every path (the waits for a slot's loads then have a fixed count of
     // later loads to leave in flight)
+    bool end_open = false, end_look = false;
     auto tile_step = [&](uint32_t t, Stage& slot, auto pf_tag) {
         constexpr bool IN_RANGE = decltype(pf_tag)::value;
         const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
@@ -748,44 +749,9 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             process(std::false_type{});
         }
 
-        if (t + 1 == t1 && !complete) {
-            // the chunk ends inside a row
-            if (look && has_row_start && (tf_next & kMultiSeg)) {
-                // the row started in this tile and ends inside tile t1: add
-                // the elements of tile t1 before its first row start
-                if constexpr (LOOK) {
-                    const W wb = bits_all[static_cast<size_t>(t1) * 64 + lane];
-                    const uint64_t any = __ballot(wb != 0);
-                    const int ls = __ffsll(static_cast<long long>(any)) - 1;
-                    const uint32_t bl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wb), ls));
-                    const int rs = __ffs(static_cast<int>(bl)) - 1;
-                    constexpr uint32_t full = SIGMA >= 32 ? ~0u : ((1u << SIGMA) - 1u);
-                    const uint32_t m = lane < ls ? full : (lane == ls ? ((1u << rs) - 1u) : 0u);
-                    const size_t b1 = static_cast<size_t>(t1) * (64 * SIGMA) + lane;
-                    V part = V(0);
-                    if (m) {
-#pragma unroll
-                        for (int r = 0; r < SIGMA; ++r)
-                            if ((m >> r) & 1u)
-                                part = fma(vals_all[b1 + r * 64], x[p.col_idx[b1 + r * 64]], part);
-                    }
-                    part = wave_sum(part);
-                    if (lane == 0) {
-                        const uint32_t row = sload(p.tile_ptr + t1) & ~kDirty;
-                        y[static_cast<size_t>(row - p.row_base)] = carry[0] + part;
-                    }
-                }
-            } else {
-                cflags |= kChunkTail;
-                if (lane == 0) {
-                    V* tl = static_cast<V*>(p.tail) + static_cast<size_t>(c) * k;
-#pragma unroll
-                    for (int j = 0; j < KMAX; ++j)
-                        if (j < k)
-                            tl[j] = carry[j];
-                }
-            }
-        }
+        // the last tile's state decides how the chunk ends (after the loop)
+        end_open = !complete;
+        end_look = look && has_row_start && (tf_next & kMultiSeg);
     };
     if constexpr (PIPE2) {
         // the main loop is entered with both slots loading on every path
@@ -808,6 +774,47 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         for (uint32_t t = t0; t < t1; ++t)
             tile_step(t, nx, std::false_type{});
     }
+    if (end_open) {
+        // the chunk ends inside a row
+        if (end_look) {
+            // the row started in the chunk's last tile and ends inside tile
+            // t1: add the elements of tile t1 before its first row start
+            if constexpr (LOOK) {
+                const W wb = bits_all[static_cast<size_t>(t1) * 64 + lane];
+                const uint64_t any = __ballot(wb != 0);
+                const int ls = __ffsll(static_cast<long long>(any)) - 1;
+                const uint32_t bl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wb), ls));
+                const int rs = __ffs(static_cast<int>(bl)) - 1;
+                constexpr uint32_t full = SIGMA >= 32 ? ~0u : ((1u << SIGMA) - 1u);
+                const uint32_t m = lane < ls ? full : (lane == ls ? ((1u << rs) - 1u) : 0u);
+                const size_t b1 = static_cast<size_t>(t1) * (64 * SIGMA) + lane;
+                V part = V(0);
+                if (m) {
+#pragma unroll
+                    for (int r = 0; r < SIGMA; ++r)
+                        if ((m >> r) & 1u)
+                            part = fma(vals_all[b1 + r * 64], x[p.col_idx[b1 + r * 64]], part);
+                }
+                part = wave_sum(part);
+                if (lane == 0) {
+                    const uint32_t row = sload(p.tile_ptr + t1) & ~kDirty;
+                    const V val = carry[0] + part;
+                    y[static_cast<size_t>(row - p.row_base)] = val;
+                    if constexpr (DOT)
+                        dacc[0] += static_cast<double>(x[row]) * static_cast<double>(val);
+                }
+            }
+        } else {
+            cflags |= kChunkTail;
+            if (lane == 0) {
+                V* tl = static_cast<V*>(p.tail) + static_cast<size_t>(c) * k;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j)
+                    if (j < k)
+                        tl[j] = carry[j];
+            }
+        }
+    }
     if (!chunk_starts && !carry_is_head && !drop_head)
         cflags |= kChunkHead;
     if (row_start_seen)
@@ -819,7 +826,23 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         for (uint32_t i = p.empty_begin + c * 64 + lane; i < p.empty_end; i += p.nchunks * 64)
             y[static_cast<size_t>(p.empty_rows[i] - p.row_base)] = V(0);
     }
-    if constexpr (DOT) {
+    if constexpr (DOT && KMAX == 1) {
+        // One partial per block of four chunks (block b: chunks 4b..4b+3, in
+        // order), one per chunk in a last block that holds fewer than four:
+        // dot_partials() of them.
+        const double s = wave_sum(dacc[0]);
+        const uint32_t base = c & ~3u;
+        if (base + 4 <= p.nchunks) { // block-uniform: every wave of the block is here
+            __shared__ double red[4];
+            if (lane == 0)
+                red[c & 3u] = s;
+            __syncthreads();
+            if (threadIdx.x == 0)
+                p.dot_chunk[base >> 2] = (red[0] + red[1]) + (red[2] + red[3]);
+        } else if (lane == 0) {
+            p.dot_chunk[(base >> 2) + (c & 3u)] = s;
+        }
+    } else if constexpr (DOT) {
 #pragma unroll
         for (int j = 0; j < KMAX; ++j) {
             const double s = wave_sum(dacc[j]);
@@ -827,6 +850,16 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 p.dot_chunk[static_cast<size_t>(c) * k + j] = s;
         }
     }
+}
+
+// Dot partials the tile kernel leaves in dot_chunk: one per chunk for blocks
+// of right-hand sides, one per block of four chunks (and per chunk of a last,
+// partial block) for single vectors.
+__host__ __device__ inline uint32_t dot_partials(int kmax, uint32_t nchunks) {
+    if (kmax != 1)
+        return nchunks;
+    const uint32_t full = nchunks / 4, rest = nchunks % 4;
+    return full + rest;
 }
 
 struct FixParams {
@@ -842,6 +875,7 @@ struct FixParams {
     const int* done;
     uint32_t tile_begin, row_base, empty_begin, empty_end;
     uint32_t tpw, nchunks;
+    uint32_t ndot;               // dot partials in dot_chunk
     int k;
     uint64_t rs, xcs, ycs;       // as TileParams::rs, xcs, ycs
     uint32_t dot_ld, dot_off;    // dot partial of column j goes to dot_blk[block*dot_ld + dot_off + j]
@@ -897,7 +931,9 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
                 }
             }
         }
-        if constexpr (DOT) {
+    }
+    if constexpr (DOT) {
+        if (c < p.ndot) {
 #pragma unroll
             for (int j = 0; j < KMAX; ++j)
                 if (j < k)
@@ -999,20 +1035,28 @@ void launch_tiles_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, 
 }
 
 // Launches the tile kernel and the fix-up kernel; records `mid` between them
-// when given. Returns the number of fix-up blocks (dot partials).
+// when given. Returns the number of dot partials per column and, in *parts,
+// where they are: one per fix-up block in fp.dot_blk, or, when a
+// single-vector product needs no fix-up, the tile kernel's partials in
+// tp.dot_chunk (dot_partials()).
 template<typename V, int KMAX>
-uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, hipStream_t st, hipEvent_t mid) {
+uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, hipStream_t st, hipEvent_t mid,
+                     const double** parts = nullptr) {
+    if (parts)
+        *parts = fp.dot_blk;
     const uint32_t ntiles = tp.tile_end - tp.tile_begin;
     const bool codes = A.col_code != nullptr;
     const Plan pl = dot ? plan_by_sigma<V, KMAX, true>(A.sigma, ntiles, codes)
                         : plan_by_sigma<V, KMAX, false>(A.sigma, ntiles, codes);
     tp.tpw = fp.tpw = pl.tpw;
     tp.nchunks = fp.nchunks = pl.nchunks;
-    // Single-vector products without a dot finish rows across chunk
-    // boundaries by look-ahead and write the empty rows themselves; the fix-up
-    // kernel then runs only when some boundary needs it.
+    fp.ndot = dot_partials(KMAX, pl.nchunks);
+    // Single-vector products finish rows across chunk boundaries by
+    // look-ahead and write the empty rows themselves; the fix-up kernel then
+    // runs only when some boundary needs it (with a dot, the chunks' partials
+    // are then left for the caller to reduce).
     bool fixup = true;
-    if (KMAX == 1 && !dot && pl.tgrid) {
+    if (KMAX == 1 && pl.tgrid) {
         tp.lookahead = 1;
         fp.empty_end = fp.empty_begin;
         fixup = !A.binfo || A.binfo->needs_fixup(A.binfo_base + tp.tile_begin, A.binfo_base + tp.tile_end, pl.tpw);
@@ -1028,12 +1072,17 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
     }
     if (mid)
         (void)hipEventRecord(mid, st);
+    if (!fixup) {
+        if (!dot)
+            return 0; // no fix-up kernel
+        if (parts)
+            *parts = tp.dot_chunk;
+        return fp.ndot;
+    }
     if (dot)
         hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, true>), dim3(pl.fgrid), dim3(256), 0, st, fp);
-    else if (fixup)
-        hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, false>), dim3(pl.fgrid), dim3(256), 0, st, fp);
     else
-        return 0; // no fix-up kernel
+        hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, false>), dim3(pl.fgrid), dim3(256), 0, st, fp);
     return pl.fgrid;
 }
 
