@@ -359,10 +359,32 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     if constexpr (PIPE)
         fetch(t0, nx);
 
+    // How the chunk ends follows from two tile flags: it ends inside a row
+    // (end_open) unless its last tile's last element ends a row; with
+    // look-ahead, the row is finished here when it started in the chunk's
+    // last tile and ends inside tile t1 (end_look). The row-start flags of
+    // tile t1, the row and, for the dot, its x entry are loaded now, so that
+    // the look-ahead after the loop waits for fewer dependent loads.
+    const uint32_t tf_last = sload_flag(p.tile_flags + t1 - 1);
+    const uint32_t tf_end = sload_flag(p.tile_flags + t1); // tile_flags holds T + 1 entries
+    const bool end_open = !((t1 == p.tile_end) || (tf_end & kStartsRow));
+    const bool end_look =
+      look && end_open && (tf_last & (kStartsRow | kMultiSeg)) != 0 && (tf_end & kMultiSeg) != 0;
+    W wb_end = 0;
+    uint32_t row_end = 0;
+    V x_end = V(0);
+    if constexpr (LOOK) {
+        if (end_look) {
+            wb_end = bits_all[static_cast<size_t>(t1) * 64 + lane];
+            row_end = sload(p.tile_ptr + t1) & ~kDirty;
+            if constexpr (DOT)
+                x_end = x[row_end];
+        }
+    }
+
     // pf_tag true: the tile t + AHEAD exists, so the prefetch is issued on
     // every path (the waits for a slot's loads then have a fixed count of
     // later loads to leave in flight)
-    bool end_open = false, end_look = false;
     auto tile_step = [&](uint32_t t, Stage& slot, auto pf_tag) {
         constexpr bool IN_RANGE = decltype(pf_tag)::value;
         const size_t base = static_cast<size_t>(t) * (64 * SIGMA);
@@ -749,9 +771,6 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             process(std::false_type{});
         }
 
-        // the last tile's state decides how the chunk ends (after the loop)
-        end_open = !complete;
-        end_look = look && has_row_start && (tf_next & kMultiSeg);
     };
     if constexpr (PIPE2) {
         // the main loop is entered with both slots loading on every path
@@ -780,7 +799,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             // the row started in the chunk's last tile and ends inside tile
             // t1: add the elements of tile t1 before its first row start
             if constexpr (LOOK) {
-                const W wb = bits_all[static_cast<size_t>(t1) * 64 + lane];
+                const W wb = wb_end;
                 const uint64_t any = __ballot(wb != 0);
                 const int ls = __ffsll(static_cast<long long>(any)) - 1;
                 const uint32_t bl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wb), ls));
@@ -797,11 +816,10 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 }
                 part = wave_sum(part);
                 if (lane == 0) {
-                    const uint32_t row = sload(p.tile_ptr + t1) & ~kDirty;
                     const V val = carry[0] + part;
-                    y[static_cast<size_t>(row - p.row_base)] = val;
+                    y[static_cast<size_t>(row_end - p.row_base)] = val;
                     if constexpr (DOT)
-                        dacc[0] += static_cast<double>(x[row]) * static_cast<double>(val);
+                        dacc[0] += static_cast<double>(x_end) * static_cast<double>(val);
                 }
             }
         } else {
