@@ -381,6 +381,45 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 x_end = x[row_end];
         }
     }
+    // The elements of tile t1 before its first row start: all of the lanes
+    // before the first lane with a row start, and that lane's elements before
+    // its first flag. Set once the flags have arrived (with the first tile's
+    // loads), before the tile loop.
+    uint32_t m_end = 0;
+    auto make_look_mask = [&]() {
+        if constexpr (LOOK) {
+            if (end_look) {
+                const uint64_t any = __ballot(wb_end != 0);
+                const int ls = __ffsll(static_cast<long long>(any)) - 1;
+                const uint32_t bl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wb_end), ls));
+                const int rs = __ffs(static_cast<int>(bl)) - 1;
+                constexpr uint32_t full = SIGMA >= 32 ? ~0u : ((1u << SIGMA) - 1u);
+                m_end = lane < ls ? full : (lane == ls ? ((1u << rs) - 1u) : 0u);
+            }
+        }
+    };
+    // Pipelined kernels load those elements (values, and column indices or
+    // the lane's code word) into the stage slot that the step of tile
+    // t1 - AHEAD would have prefetched into, so that after the loop only
+    // their x gathers remain.
+    auto fetch_look = [&](Stage& s) {
+        if constexpr (PIPE) {
+            const size_t tile64 = static_cast<size_t>(__builtin_amdgcn_readfirstlane(t1)) * 64;
+            const size_t b1 = tile64 * SIGMA;
+#pragma unroll
+            for (int r = 0; r < SIGMA; ++r) {
+                if ((m_end >> r) & 1u) {
+                    s.v[r] = vals_all[b1 + r * 64 + ulane];
+                    if constexpr (!CODED)
+                        s.col[r] = p.col_idx[b1 + r * 64 + ulane];
+                }
+            }
+            if constexpr (CODED) {
+                if (m_end)
+                    s.code = p.col_code[tile64 + ulane];
+            }
+        }
+    };
 
     // pf_tag true: the tile t + AHEAD exists, so the prefetch is issued on
     // every path (the waits for a slot's loads then have a fixed count of
@@ -410,6 +449,8 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             __builtin_amdgcn_sched_barrier(0);
             if (IN_RANGE || t + AHEAD < t1)
                 fetch(t + AHEAD, slot);
+            else if (LOOK && end_look && t + AHEAD == t1)
+                fetch_look(slot);
         } else {
             fetch(t, cu);
             decode(cu);
@@ -777,12 +818,15 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         uint32_t t = t0;
         if (t0 + 3 < t1) {
             fetch(t0 + 1, nx2);
+            make_look_mask();
             for (; t + 3 < t1; t += 2) {
                 tile_step(t, nx, std::true_type{});
                 tile_step(t + 1, nx2, std::true_type{});
             }
-        } else if (t0 + 1 < t1) {
-            fetch(t0 + 1, nx2);
+        } else {
+            if (t0 + 1 < t1)
+                fetch(t0 + 1, nx2);
+            make_look_mask();
         }
         for (; t < t1; t += 2) {
             tile_step(t, nx, std::false_type{});
@@ -790,6 +834,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 tile_step(t + 1, nx2, std::false_type{});
         }
     } else {
+        make_look_mask();
         for (uint32_t t = t0; t < t1; ++t)
             tile_step(t, nx, std::false_type{});
     }
@@ -799,19 +844,41 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             // the row started in the chunk's last tile and ends inside tile
             // t1: add the elements of tile t1 before its first row start
             if constexpr (LOOK) {
-                const W wb = wb_end;
-                const uint64_t any = __ballot(wb != 0);
-                const int ls = __ffsll(static_cast<long long>(any)) - 1;
-                const uint32_t bl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wb), ls));
-                const int rs = __ffs(static_cast<int>(bl)) - 1;
-                constexpr uint32_t full = SIGMA >= 32 ? ~0u : ((1u << SIGMA) - 1u);
-                const uint32_t m = lane < ls ? full : (lane == ls ? ((1u << rs) - 1u) : 0u);
-                const size_t b1 = static_cast<size_t>(t1) * (64 * SIGMA) + lane;
                 V part = V(0);
-                if (m) {
+                auto look_sum = [&](const Stage& st) {
+                    // every element here lies in row row_end: a coded
+                    // element's column is that row plus its diagonal
+                    uint32_t cl[SIGMA];
+#pragma unroll
+                    for (int r = 0; r < SIGMA; ++r) {
+                        if constexpr (CODED) {
+                            constexpr uint32_t CB = code_bits(SIGMA);
+                            const int d = __builtin_amdgcn_ds_bpermute(
+                              static_cast<int>((st.code >> (CB * r)) & ((1u << CB) - 1u)) << 2, dtab);
+                            cl[r] = min(row_end + static_cast<uint32_t>(d), p.cols - 1u);
+                        } else {
+                            cl[r] = st.col[r];
+                        }
+                    }
+                    if (m_end) {
+#pragma unroll
+                        for (int r = 0; r < SIGMA; ++r)
+                            if ((m_end >> r) & 1u)
+                                part = fma(st.v[r], x[cl[r]], part);
+                    }
+                };
+                // the step of tile t1 - AHEAD loaded them when it exists
+                // (slot nx at an even offset from t0, nx2 at an odd one)
+                if (PIPE && t1 >= t0 + AHEAD) {
+                    if (PIPE2 && ((t1 - AHEAD - t0) & 1u))
+                        look_sum(nx2);
+                    else
+                        look_sum(nx);
+                } else if (m_end) {
+                    const size_t b1 = static_cast<size_t>(t1) * (64 * SIGMA) + lane;
 #pragma unroll
                     for (int r = 0; r < SIGMA; ++r)
-                        if ((m >> r) & 1u)
+                        if ((m_end >> r) & 1u)
                             part = fma(vals_all[b1 + r * 64], x[p.col_idx[b1 + r * 64]], part);
                 }
                 part = wave_sum(part);

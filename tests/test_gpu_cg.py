@@ -36,13 +36,15 @@ def test_cg_poisson2d_256_matches_oracle(cuda, sigma):
     np.testing.assert_allclose(x.cpu().numpy(), xs, atol=1e-7)
 
 
-def test_cg_poisson3d_128_full_size(cuda):
+@pytest.mark.parametrize("sigma", [16, 7])
+def test_cg_poisson3d_128_full_size(cuda, sigma):
     """The benchmark operator: 128^3 7-point Poisson, b = A x*, rtol 1e-8,
-    max 5000 iterations: same iteration count as the oracle within +-2."""
+    max 5000 iterations: same iteration count as the oracle within +-2
+    (sigma 7: the column-coded, two-deep pipelined product with look-ahead)."""
     rp, ci, v = matgen.poisson3d(128)
     n = 128 ** 3
     xs, b = _problem(rp, ci, v, n, 42)
-    A = Csr5.from_csr((n, n, rp, ci, v), sigma=16)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=sigma)
     x, res = conjugate_gradient(A, to_dev(b), rtol=1e-8, max_iters=5000)
     oc = O.Csr5(n, n, rp, ci, v).cg(b, rtol=1e-8, maxit=5000, nthreads=16)
     assert res.converged and oc["converged"]
@@ -50,6 +52,23 @@ def test_cg_poisson3d_128_full_size(cuda):
     assert res.relres_true <= 2e-8
     err = np.linalg.norm(x.cpu().numpy() - oc["x"]) / np.linalg.norm(oc["x"])
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("n", [73, 87])
+def test_cg_coded_products_of_short_chunks(cuda, n):
+    """CG's product with the fused p.q on grids whose chunks hold two and
+    three tiles (look-ahead elements loaded into either pipeline slot):
+    iteration count within +-2 of the oracle's CG."""
+    rp, ci, v = matgen.poisson3d(n)
+    rows = n ** 3
+    xs, b = _problem(rp, ci, v, rows, n)
+    A = Csr5.from_csr((rows, rows, rp, ci, v), sigma=7)
+    assert A.info.column_codes == 1
+    x, res = conjugate_gradient(A, to_dev(b), rtol=1e-8, max_iters=5000)
+    oc = O.Csr5(rows, rows, rp, ci, v).cg(b, rtol=1e-8, maxit=5000, nthreads=16)
+    assert res.converged and oc["converged"]
+    assert abs(res.iterations - oc["iterations"]) <= 2, (res.iterations, oc["iterations"])
+    assert res.relres_true <= 2e-8
 
 
 def test_cg_is_bitwise_reproducible(cuda):

@@ -318,6 +318,32 @@ def test_column_codes_equal_plain_columns(cuda, sigma):
             assert fa == fb and torch.equal(ya, yb)
 
 
+@pytest.mark.parametrize("sigma", [4, 5, 6, 7, 8])
+def test_lookahead_across_chunk_lengths(cuda, sigma):
+    """A row that crosses from a chunk's last tile into the next chunk's
+    first tile is finished by the earlier chunk (look-ahead); the pipelined
+    kernels load its elements in tile t1 during the chunk's last steps, into
+    the stage slot that no prefetch needs any more. The grids here give
+    chunks of one to five tiles (odd and even counts, so both slots of the
+    two-deep pipeline), with column codes and with 32-bit columns; integer
+    inputs must give the oracle's result exactly, FP64 and FP32."""
+    import torch
+
+    for n in (60, 73, 87, 101):
+        rp, ci, _ = matgen.poisson3d(n)
+        rows = n ** 3
+        rng = np.random.default_rng(n * 10 + sigma)
+        v = rng.integers(1, 5, len(ci)).astype(np.float64) * rng.choice([-1.0, 1.0], len(ci))
+        xi = rng.integers(-5, 6, rows).astype(np.float64)
+        expect = O.csr_spmv(rp, ci, v, xi)
+        for plain in (False, True):
+            A = Csr5.from_csr((rows, rows, rp, ci, v), sigma=sigma, keep_f32=True, plain_columns=plain)
+            assert A.info.column_codes == (0 if plain else 1)
+            np.testing.assert_array_equal(A.spmv(to_dev(xi)).cpu().numpy(), expect, err_msg=f"n={n} plain={plain}")
+            y32 = A.spmv(to_dev(xi, torch.float32)).cpu().numpy()
+            np.testing.assert_array_equal(y32.astype(np.float64), expect, err_msg=f"f32 n={n} plain={plain}")
+
+
 @pytest.mark.parametrize("shape", ["wide", "tiny", "one_row"])
 def test_column_codes_rectangular_and_small(cuda, shape):
     """Codes on rectangular banded matrices (diagonals far off the main one)
