@@ -174,15 +174,16 @@ def bench_mixed(side=192, k=8, rtol=1e-10):
     return out
 
 
-def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-3, rtol=1e-13):
+def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-6, h4=1e-3, rtol=1e-13):
     """Uneven batch of 2-D variable-coefficient systems (sides uniform in
     [lo, hi], lognormal(0, 0.5)) solved in one batched FP64 CG launch
     sequence; L = 0.5 ||x||^2; dL/db and dL/dvals from the adjoint (transpose
-    CG + sampled outer product) through torch.autograd; 4th-order central
-    differences (step h) on n_fd random stored entries, each four re-solves of
-    its system. The step is large enough that the solves' rounding (rtol
-    1e-15) stays far below the 1e-6 bar, and the stencil's truncation error
-    (of order h^4) too."""
+    CG + sampled outer product) through torch.autograd; central differences
+    on n_fd random stored entries, each a set of re-solves of its system:
+    2nd order with step h, and 4th order with step h4. With h = 1e-6 the
+    solves' rounding (rtol 1e-15) shows in the quotient at about 1e-6 for
+    some entries; with h4 = 1e-3 it stays far below, and the 4th-order
+    stencil's truncation error (of order h4^4) too."""
     import torch
 
     from diploma_research_amd import Csr5, conjugate_gradient, matgen
@@ -215,7 +216,7 @@ def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-3, rtol=1e-13)
     starts = np.concatenate([[0], np.cumsum([len(v) for *_, v in systems])])
     picks = rng.choice(len(vals_np), size=n_fd, replace=False)
     singles = {}
-    errs = []
+    errs, errs4 = [], []
     t3 = time.perf_counter()
     for e in picks:
         s = int(np.searchsorted(starts, e, side="right") - 1)
@@ -232,28 +233,33 @@ def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-3, rtol=1e-13)
             mirror.append(int(rp[j]) + int(np.nonzero(ci[seg] == i)[0][0]))
         bd = torch.from_numpy(bs[s]).cuda()
         xs = {}
-        for step in (2.0, 1.0, -1.0, -2.0):
+        for step in (2 * h4, h4, -h4, -2 * h4, h, -h):
             vv = v.copy()
             for m in mirror:
-                vv[m] += step * h
+                vv[m] += step
             M.set_values(torch.from_numpy(vv).cuda())
             xs[step], _ = conjugate_gradient(M, bd, rtol=1e-15, max_iters=50000)
 
         def diff(a, b):  # L(a) - L(b) without cancelling 0.5||x||^2 terms
             return 0.5 * float(torch.dot(xs[a] - xs[b], xs[a] + xs[b]))
 
-        fd = (8.0 * diff(1.0, -1.0) - diff(2.0, -2.0)) / (12.0 * h)
+        fd2 = diff(h, -h) / (2 * h)
+        fd4 = (8.0 * diff(h4, -h4) - diff(2 * h4, -2 * h4)) / (12.0 * h4)
         an = float(sum(gv[starts[s] + m] for m in mirror))
-        errs.append(abs(fd - an) / max(abs(an), 1e-300))
+        errs.append(abs(fd2 - an) / max(abs(an), 1e-300))
+        errs4.append(abs(fd4 - an) / max(abs(an), 1e-300))
     t4 = time.perf_counter()
-    errs = np.array(errs)
+    errs, errs4 = np.array(errs), np.array(errs4)
     return {"workload": f"{n_sys} 2-D variable-coefficient systems, sides {int(sides.min())}..{int(sides.max())} "
                         f"(n={int(sum(n for n, *_ in systems))}, nnz={len(vals_np)}), lognormal(0,0.5), FP64, "
                         f"rtol {rtol:g}",
             "sigma": A.sigma, "forward_seconds": round(t1 - t0, 4), "backward_seconds": round(t2 - t1, 4),
-            "fd": {"entries": n_fd, "h": h, "stencil": "central, 4th order", "seconds": round(t4 - t3, 2),
+            "fd": {"entries": n_fd, "h": h, "stencil": "central, 2nd order", "seconds": round(t4 - t3, 2),
                    "rel_err_median": float(np.median(errs)), "rel_err_max": float(errs.max()),
-                   "within_1e-6": int((errs <= 1e-6).sum())}}
+                   "within_1e-6": int((errs <= 1e-6).sum())},
+            "fd4": {"entries": n_fd, "h": h4, "stencil": "central, 4th order",
+                    "rel_err_median": float(np.median(errs4)), "rel_err_max": float(errs4.max()),
+                    "within_1e-6": int((errs4 <= 1e-6).sum())}}
 
 
 def main():
