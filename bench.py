@@ -177,8 +177,9 @@ def bench_mixed(side=192, k=8, rtol=1e-10):
 def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-6, h4=1e-3, rtol=1e-13):
     """Uneven batch of 2-D variable-coefficient systems (sides uniform in
     [lo, hi], lognormal(0, 0.5)) solved in one batched FP64 CG launch
-    sequence; L = 0.5 ||x||^2; dL/db and dL/dvals from the adjoint (transpose
-    CG + sampled outer product) through torch.autograd; central differences
+    sequence; L = 0.5 ||x||^2; dL/db and dL/dvals from the adjoint (batched CG
+    with A's own product, A being symmetric, + sampled outer product) through
+    torch.autograd; central differences
     on n_fd random stored entries, each a set of re-solves of its system:
     2nd order with step h, and 4th order with step h4. With h = 1e-6 the
     solves' rounding (rtol 1e-15) shows in the quotient at about 1e-6 for

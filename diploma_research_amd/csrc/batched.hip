@@ -209,14 +209,18 @@ __global__ void k_b_init_state(BState* st, SysArrays sa, Tables tb, const double
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_b_dot(const BState* st, Tables tb, const double* p, const double* q,
-                                                    double* pq_part) {
+// Systems that have stopped skip their vector work: their alpha is 0, so x
+// and r stay as they are, and their partials are not used.
+__global__ __launch_bounds__(kThreads) void k_b_dot(const BState* st, SysArrays sa, Tables tb, const double* p,
+                                                    const double* q, double* pq_part) {
     if (st->done)
         return;
-    const uint32_t r0 = tb.blk_r0[blockIdx.x], r1 = tb.blk_r1[blockIdx.x];
     double acc = 0.0;
-    for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
-        acc += p[i] * q[i];
+    if (!sa.sys_done[tb.blk_sys[blockIdx.x]]) {
+        const uint32_t r0 = tb.blk_r0[blockIdx.x], r1 = tb.blk_r1[blockIdx.x];
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
+            acc += p[i] * q[i];
+    }
     block_store_sum(acc, pq_part);
 }
 
@@ -227,14 +231,19 @@ __global__ __launch_bounds__(kThreads) void k_b_update_xr(BState* st, SysArrays 
         return;
     const uint32_t s = tb.blk_sys[blockIdx.x];
     __shared__ double pq, rr, alpha;
+    __shared__ int skip;
     range_sum(pq_part, tb.sys_blk[s], tb.sys_blk[s + 1], &pq);
     range_sum(rr_in, tb.sys_blk[s], tb.sys_blk[s + 1], &rr);
     if (threadIdx.x == 0) {
         double a = rr / pq;
         const bool broke = !(pq > 0.0) || !std::isfinite(a);
-        if (sa.sys_done[s] || broke)
+        // every block of the system decides alike: a stopped system (or one
+        // breaking down now, which its first block records) keeps x and r
+        const bool stop = sa.sys_done[s] || broke;
+        if (stop)
             a = 0.0;
         alpha = a;
+        skip = stop;
         if (broke && !sa.sys_done[s] && blockIdx.x == tb.sys_blk[s]) {
             sa.sys_done[s] = 2;
             sa.sys_iters[s] = st->iterations;
@@ -242,13 +251,15 @@ __global__ __launch_bounds__(kThreads) void k_b_update_xr(BState* st, SysArrays 
     }
     __syncthreads();
     const double al = alpha;
-    const uint32_t r0 = tb.blk_r0[blockIdx.x], r1 = tb.blk_r1[blockIdx.x];
     double acc = 0.0;
-    for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        x[i] = x[i] + al * p[i];
-        const double ri = r[i] - al * q[i];
-        r[i] = ri;
-        acc += ri * ri;
+    if (!skip) {
+        const uint32_t r0 = tb.blk_r0[blockIdx.x], r1 = tb.blk_r1[blockIdx.x];
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+            x[i] = x[i] + al * p[i];
+            const double ri = r[i] - al * q[i];
+            r[i] = ri;
+            acc += ri * ri;
+        }
     }
     block_store_sum(acc, rr_out);
 }
@@ -262,8 +273,8 @@ __global__ __launch_bounds__(kThreads) void k_b_update_p(BState* st, SysArrays s
     __shared__ double rn, ro;
     range_sum(rr_new_part, tb.sys_blk[s], tb.sys_blk[s + 1], &rn);
     range_sum(rr_old_part, tb.sys_blk[s], tb.sys_blk[s + 1], &ro);
-    // frozen systems keep r, so their test repeats with the same numbers
-    const bool conv = rn <= sa.tol2[s];
+    // a stopped system left its r.r partials at 0 and keeps p
+    const bool conv = sa.sys_done[s] || rn <= sa.tol2[s];
     const double beta = rn / ro;
     if (!conv) {
         const uint32_t r0 = tb.blk_r0[blockIdx.x], r1 = tb.blk_r1[blockIdx.x];
@@ -279,7 +290,10 @@ __global__ __launch_bounds__(kThreads) void k_b_update_p(BState* st, SysArrays s
     for (int q = 0; q < st->n_sys; ++q) {
         range_sum(rr_new_part, tb.sys_blk[q], tb.sys_blk[q + 1], &rs);
         if (threadIdx.x == 0) {
-            sa.rr[q] = rs;
+            // a stopped system keeps the r.r of its last step (its r no
+            // longer changes)
+            if (!sa.sys_done[q])
+                sa.rr[q] = rs;
             if (!sa.sys_done[q] && rs <= sa.tol2[q]) {
                 sa.sys_done[q] = 1;
                 sa.sys_iters[q] = it;
@@ -398,7 +412,7 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
             double* rin = (it & 1) ? rr1 : rr0;
             double* rout = (it & 1) ? rr0 : rr1;
             (void)product(A, transpose, p, q, &st->done, s);
-            hipLaunchKernelGGL(k_b_dot, dim3(B), dim3(kThreads), 0, s, st, tb, static_cast<const double*>(p),
+            hipLaunchKernelGGL(k_b_dot, dim3(B), dim3(kThreads), 0, s, st, sa, tb, static_cast<const double*>(p),
                                static_cast<const double*>(q), pq);
             hipLaunchKernelGGL(k_b_update_xr, dim3(B), dim3(kThreads), 0, s, st, sa, tb, x, r,
                                static_cast<const double*>(p), static_cast<const double*>(q),

@@ -97,10 +97,12 @@ def _loss_dense(systems, vals_list, bs):
     return L
 
 
-def test_adjoint_gradient_matches_dense_reference_and_finite_differences(cuda):
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_adjoint_gradient_matches_dense_reference_and_finite_differences(cuda, symmetric):
     """L = 0.5 ||x||^2, x = A^-1 b over a batch of uneven 2-D variable-
-    coefficient systems. dL/db and dL/dvals from the adjoint (transpose CG +
-    sampled outer product) against (1) the dense FP64 reference gradient and
+    coefficient systems. dL/db and dL/dvals from the adjoint (CG with A's own
+    products, or with the transposed kernel when symmetric=False, + sampled
+    outer product) against (1) the dense FP64 reference gradient and
     (2) fourth-order central finite differences, h = 1e-3, on 64 random
     entries, perturbing a_ij and a_ji together (the systems are symmetric, so
     the gradient of that parametrisation is g_ij + g_ji)."""
@@ -116,7 +118,7 @@ def test_adjoint_gradient_matches_dense_reference_and_finite_differences(cuda):
     vals_np = np.concatenate([v for *_, v in systems])
     vals = torch.from_numpy(vals_np.copy()).cuda().requires_grad_(True)
     b = torch.from_numpy(np.concatenate(bs)).cuda().requires_grad_(True)
-    x = solve(A, vals, b, rtol=1e-14, max_iters=20000)
+    x = solve(A, vals, b, rtol=1e-14, max_iters=20000, symmetric=symmetric)
     L = 0.5 * (x * x).sum()
     L.backward()
     gv = vals.grad.cpu().numpy()
