@@ -283,13 +283,19 @@ __global__ __launch_bounds__(kThreads) void k_b_update_p(BState* st, SysArrays s
     }
     if (blockIdx.x != 0)
         return;
-    // block 0 keeps the batch state: every system's test, in order
-    __shared__ double rs;
-    int all = 1;
+    // Block 0 keeps the batch state. Wave w tests systems w, w + 16, ...; each
+    // sum runs in range_sum's order (lane l adds entries l, l + 64, ..., then
+    // the butterfly), so the bits do not depend on which wave sums.
+    __shared__ int wave_all[kThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int it = st->iterations + 1;
-    for (int q = 0; q < st->n_sys; ++q) {
-        range_sum(rr_new_part, tb.sys_blk[q], tb.sys_blk[q + 1], &rs);
-        if (threadIdx.x == 0) {
+    int all = 1;
+    for (int q = w; q < st->n_sys; q += nw) {
+        double acc = 0.0;
+        for (uint32_t i = tb.sys_blk[q] + lane; i < tb.sys_blk[q + 1]; i += 64)
+            acc += rr_new_part[i];
+        const double rs = wave_sum(acc);
+        if (lane == 0) {
             // a stopped system keeps the r.r of its last step (its r no
             // longer changes)
             if (!sa.sys_done[q])
@@ -300,9 +306,13 @@ __global__ __launch_bounds__(kThreads) void k_b_update_p(BState* st, SysArrays s
             }
             all &= sa.sys_done[q] != 0;
         }
-        __syncthreads();
     }
+    if (lane == 0)
+        wave_all[w] = all;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int i = 0; i < nw; ++i)
+            all &= wave_all[i];
         st->iterations = it;
         if (all)
             st->done = 1;
