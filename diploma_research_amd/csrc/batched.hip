@@ -122,6 +122,10 @@ struct BState {
     int iterations;
     int max_iters;
     int n_sys;
+    // `done` as k_b_update_r saw it. k_b_update_xp tests this copy: its own
+    // block 0 sets `done` at its end, and a block of the same launch that
+    // starts after that must still apply this iteration's x update.
+    int done_prev;
 };
 
 // per-system arrays (n_sys entries each)
@@ -131,6 +135,7 @@ struct SysArrays {
     double* tol2;
     double* bnorm;
     double* rr;
+    double* alpha;   // this iteration's step (0 for a stopped system)
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -224,9 +229,14 @@ __global__ __launch_bounds__(kThreads) void k_b_dot(const BState* st, SysArrays 
     block_store_sum(acc, pq_part);
 }
 
-__global__ __launch_bounds__(kThreads) void k_b_update_xr(BState* st, SysArrays sa, Tables tb, double* x, double* r,
-                                                          const double* p, const double* q, const double* pq_part,
-                                                          const double* rr_in, double* rr_out) {
+// alpha = r.r / p.q per system; r -= alpha q; r.r partials. The step goes
+// to sa.alpha for k_b_update_xp, which moves x with the same p that p.q
+// used, in its pass over p (x is then read and written once per iteration).
+__global__ __launch_bounds__(kThreads) void k_b_update_r(BState* st, SysArrays sa, Tables tb, double* r,
+                                                         const double* q, const double* pq_part, const double* rr_in,
+                                                         double* rr_out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        st->done_prev = st->done;
     if (st->done)
         return;
     const uint32_t s = tb.blk_sys[blockIdx.x];
@@ -244,9 +254,12 @@ __global__ __launch_bounds__(kThreads) void k_b_update_xr(BState* st, SysArrays 
             a = 0.0;
         alpha = a;
         skip = stop;
-        if (broke && !sa.sys_done[s] && blockIdx.x == tb.sys_blk[s]) {
-            sa.sys_done[s] = 2;
-            sa.sys_iters[s] = st->iterations;
+        if (blockIdx.x == tb.sys_blk[s]) {
+            sa.alpha[s] = a;
+            if (broke && !sa.sys_done[s]) {
+                sa.sys_done[s] = 2;
+                sa.sys_iters[s] = st->iterations;
+            }
         }
     }
     __syncthreads();
@@ -255,7 +268,6 @@ __global__ __launch_bounds__(kThreads) void k_b_update_xr(BState* st, SysArrays 
     if (!skip) {
         const uint32_t r0 = tb.blk_r0[blockIdx.x], r1 = tb.blk_r1[blockIdx.x];
         for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-            x[i] = x[i] + al * p[i];
             const double ri = r[i] - al * q[i];
             r[i] = ri;
             acc += ri * ri;
@@ -264,22 +276,28 @@ __global__ __launch_bounds__(kThreads) void k_b_update_xr(BState* st, SysArrays 
     block_store_sum(acc, rr_out);
 }
 
-__global__ __launch_bounds__(kThreads) void k_b_update_p(BState* st, SysArrays sa, Tables tb, const double* r,
-                                                         double* p, const double* rr_new_part,
-                                                         const double* rr_old_part) {
-    if (st->done)
+// x += alpha p and p = r + beta p in one pass (p read once); stop tests.
+__global__ __launch_bounds__(kThreads) void k_b_update_xp(BState* st, SysArrays sa, Tables tb, double* x,
+                                                          const double* r, double* p, const double* rr_new_part,
+                                                          const double* rr_old_part) {
+    if (st->done_prev)
         return;
     const uint32_t s = tb.blk_sys[blockIdx.x];
     __shared__ double rn, ro;
     range_sum(rr_new_part, tb.sys_blk[s], tb.sys_blk[s + 1], &rn);
     range_sum(rr_old_part, tb.sys_blk[s], tb.sys_blk[s + 1], &ro);
+    const double al = sa.alpha[s];
     // a stopped system left its r.r partials at 0 and keeps p
     const bool conv = sa.sys_done[s] || rn <= sa.tol2[s];
     const double beta = rn / ro;
-    if (!conv) {
+    if (al != 0.0 || !conv) {
         const uint32_t r0 = tb.blk_r0[blockIdx.x], r1 = tb.blk_r1[blockIdx.x];
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x)
-            p[i] = r[i] + beta * p[i];
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+            const double pi = p[i];
+            x[i] = x[i] + al * pi;
+            if (!conv)
+                p[i] = r[i] + beta * pi;
+        }
     }
     if (blockIdx.x != 0)
         return;
@@ -403,6 +421,7 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
     DIM_HIP_TRY(m.alloc(sa.tol2, S));
     DIM_HIP_TRY(m.alloc(sa.bnorm, S));
     DIM_HIP_TRY(m.alloc(sa.rr, S));
+    DIM_HIP_TRY(m.alloc(sa.alpha, S));
     DIM_HIP_TRY(m.alloc(q, n));
     DIM_HIP_TRY(m.alloc(r, n));
     DIM_HIP_TRY(m.alloc(p, n));
@@ -424,11 +443,12 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
             (void)product(A, transpose, p, q, &st->done, s);
             hipLaunchKernelGGL(k_b_dot, dim3(B), dim3(kThreads), 0, s, st, sa, tb, static_cast<const double*>(p),
                                static_cast<const double*>(q), pq);
-            hipLaunchKernelGGL(k_b_update_xr, dim3(B), dim3(kThreads), 0, s, st, sa, tb, x, r,
-                               static_cast<const double*>(p), static_cast<const double*>(q),
-                               static_cast<const double*>(pq), static_cast<const double*>(rin), rout);
-            hipLaunchKernelGGL(k_b_update_p, dim3(B), dim3(kThreads), 0, s, st, sa, tb, static_cast<const double*>(r), p,
-                               static_cast<const double*>(rout), static_cast<const double*>(rin));
+            hipLaunchKernelGGL(k_b_update_r, dim3(B), dim3(kThreads), 0, s, st, sa, tb, r,
+                               static_cast<const double*>(q), static_cast<const double*>(pq),
+                               static_cast<const double*>(rin), rout);
+            hipLaunchKernelGGL(k_b_update_xp, dim3(B), dim3(kThreads), 0, s, st, sa, tb, x,
+                               static_cast<const double*>(r), p, static_cast<const double*>(rout),
+                               static_cast<const double*>(rin));
         }
     };
     bool use_graph = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;

@@ -53,6 +53,10 @@ struct CgState {
     double rr_true[kMaxK]; // FP64 residual norm^2 at the last residual replacement
     double alpha[kMaxK];   // this iteration's step, from k_cg_update_r to k_cg_update_xp
     int broke;             // a column broke down in this iteration
+    // `done` as k_cg_update_r saw it. k_cg_update_xp tests this copy: its own
+    // block (0, 0) sets `done` at its end, and a block of the same launch
+    // that starts after that must still apply this iteration's x update.
+    int done_prev;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -181,6 +185,8 @@ template<typename V>
 __global__ __launch_bounds__(kVecThreads) void k_cg_update_r(CgState* st, size_t n, V* r, const V* q,
                                                              const double* pq_part, int n_pq, const double* rr_in,
                                                              double* rr_out, int n_rr) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+        st->done_prev = st->done;
     if (st->done)
         return;
     const int j = blockIdx.y, k = gridDim.y;
@@ -221,7 +227,7 @@ template<typename V, int KMAX>
 __global__ __launch_bounds__(kVecThreads) void k_cg_update_xp(CgState* st, size_t n, V* x, const V* r, V* p,
                                                               const double* rr_new_part, const double* rr_old_part,
                                                               int n_rr, double* hist, int hist_cap) {
-    if (st->done)
+    if (st->done_prev)
         return;
     const int j = blockIdx.y, k = gridDim.y;
     __shared__ double rn, ro;
