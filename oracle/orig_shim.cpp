@@ -4,8 +4,10 @@
 // reference tree into oracle/_ref/libcsr5orig.so.
 //
 // TEST INFRASTRUCTURE ONLY: a second CPU oracle that cross-checks the
-// restatement of the reference (its CSR5 descends from this code; SURVEY
-// Appendix A lists where the two differ). It is never part of the product.
+// restatement of the reference (its CSR5 descends from this code; the two
+// differ in the tile count, where the original keeps a tail partition, and
+// in how the first row of a tile is written). It is never part of the
+// product.
 #include <anonymouslib_avx2.h>
 
 #include <fcntl.h>
@@ -48,7 +50,7 @@ extern "C" {
 
 // y = A x with the original CSR5 at tile height sigma. The conversion
 // transposes its own copies of the column and value arrays; y is zero-filled
-// first because the original never writes empty rows (SURVEY Appendix A.4).
+// first because the original never writes empty rows.
 // Returns the library's status (0 = success).
 int csr5orig_spmv(int rows, int cols, int nnz, const int* row_ptr, const int* col_idx, const double* vals,
                   int sigma, const double* x, double* y) {
