@@ -45,6 +45,27 @@ def test_batched_cg_matches_oracle_per_system(cuda, sigma):
         off += n
 
 
+def test_batched_cg_more_blocks_than_resident(cuda):
+    """Three large systems (2.3M rows, 563 blocks of 1024 threads, more than
+    the GPU holds at once): blocks of the last x/p pass start after its block
+    0 has set the stop flag, and must still take their last x update. Each
+    system matches its own single-system solve."""
+    systems = _systems([900, 700, 1000], 0.5, seed=9)
+    A = Csr5.from_csr_batched(systems, sigma=0)
+    rng = np.random.default_rng(3)
+    bs = [rng.uniform(-1, 1, n) for n, *_ in systems]
+    X, res = conjugate_gradient_batched(A, to_dev(np.concatenate(bs)), rtol=1e-10, max_iters=20000)
+    X = X.cpu().numpy()
+    off = 0
+    for (n, rp, ci, v), b, r in zip(systems, bs, res):
+        M = Csr5.from_csr((n, n, rp, ci, v), sigma=A.sigma)
+        xs, rs = conjugate_gradient(M, to_dev(b), rtol=1e-10, max_iters=20000)
+        assert r.converged and abs(r.iterations - rs.iterations) <= 2, (r.iterations, rs.iterations)
+        assert r.relres_true <= 1.01e-10, r.relres_true
+        np.testing.assert_allclose(X[off: off + n], xs.cpu().numpy(), rtol=1e-7, atol=1e-8)
+        off += n
+
+
 def test_batched_products_and_values(cuda):
     import torch
 

@@ -152,6 +152,25 @@ def test_cg_multi_rhs_equals_single(cuda, k):
         np.testing.assert_allclose(X[:, j].cpu().numpy(), xj.cpu().numpy(), rtol=1e-7, atol=1e-9)
 
 
+def test_cg_multi_rhs_more_blocks_than_resident(cuda):
+    """96^3 Poisson with 8 right-hand sides: the vector passes run 1728
+    blocks of 1024 threads, more than the GPU holds at once, so blocks of
+    the last x/p pass start after its block (0, 0) has set the stop flag.
+    Every column must still take its last x update: true residuals at the
+    tolerance, and each column equal to its single-RHS solve."""
+    rp, ci, v = matgen.poisson3d(96)
+    n = 96 ** 3
+    k = 8
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=7)
+    B = np.random.default_rng(96).uniform(-1, 1, (n, k))
+    X, res = conjugate_gradient_multi(A, to_dev(B), rtol=1e-10, max_iters=3000)
+    for j in range(k):
+        xj, rj = conjugate_gradient(A, to_dev(B[:, j].copy()), rtol=1e-10, max_iters=3000)
+        assert res[j].converged and abs(res[j].iterations - rj.iterations) <= 2
+        assert res[j].relres_true <= 1.01e-10, (j, res[j].relres_true)
+        np.testing.assert_allclose(X[:, j].cpu().numpy(), xj.cpu().numpy(), rtol=1e-7, atol=1e-9)
+
+
 @pytest.mark.parametrize("k", [1, 4])
 def test_cg_fp32_refined_reaches_fp64_accuracy(cuda, k):
     """FP32 iteration with FP64 residual replacement reaches relative
