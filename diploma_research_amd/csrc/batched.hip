@@ -375,10 +375,13 @@ struct DevMem {
     }
 };
 
-hipError_t product(dim_csr5* A, bool transpose, const double* x, double* y, const int* done, hipStream_t s) {
+// tile_sys, sys_done (may be null): chunks of stopped systems are skipped.
+hipError_t product(dim_csr5* A, bool transpose, const double* x, double* y, const int* done, hipStream_t s,
+                   const uint32_t* tile_sys = nullptr, const int* sys_done = nullptr) {
     if (transpose)
         return launch_spmv_transpose(A->f, false, x, y, s, done);
-    return launch_spmv(A->f, A->s, false, 1, x, y, 0, A->f.tiles, 0, 0, A->f.n_empty, false, done, s);
+    return launch_spmv(A->f, A->s, false, 1, x, y, 0, A->f.tiles, 0, 0, A->f.n_empty, false, done, s, nullptr, 0, 0,
+                       nullptr, tile_sys, sys_done);
 }
 
 dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_opts& o, dim_cg_result* res,
@@ -412,6 +415,13 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
     tb.blk_r0 = d_r0;
     tb.blk_r1 = d_r1;
     tb.sys_blk = d_sb;
+    // the system of each tile, for the product's skip of stopped systems
+    std::vector<uint32_t> tile_sys(A->f.tiles);
+    for (uint32_t s = 0; s < S; ++s)
+        for (uint32_t t = A->sys_tile[s]; t < A->sys_tile[s + 1]; ++t)
+            tile_sys[t] = s;
+    uint32_t* d_ts = nullptr;
+    DIM_HIP_TRY(m.upload(d_ts, tile_sys));
     BState* st;
     SysArrays sa{};
     double *q, *r, *p, *rr0, *rr1, *pq, *bb, *rt;
@@ -440,7 +450,7 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
         for (uint32_t it = 0; it < check; ++it) {
             double* rin = (it & 1) ? rr1 : rr0;
             double* rout = (it & 1) ? rr0 : rr1;
-            (void)product(A, transpose, p, q, &st->done, s);
+            (void)product(A, transpose, p, q, &st->done, s, d_ts, sa.sys_done);
             hipLaunchKernelGGL(k_b_dot, dim3(B), dim3(kThreads), 0, s, st, sa, tb, static_cast<const double*>(p),
                                static_cast<const double*>(q), pq);
             hipLaunchKernelGGL(k_b_update_r, dim3(B), dim3(kThreads), 0, s, st, sa, tb, r,

@@ -75,6 +75,10 @@ struct Scratch {
 // partial per fix-up block, or scratch.dot_chunk, one per chunk, when a
 // single-vector product needs no fix-up kernel) and *dot_blocks (if given)
 // the number of partials.
+// tile_sys, sys_done (single-vector products without a dot; may be null):
+// the system of each tile of a block-diagonal batch and the systems' stop
+// flags; chunks whose tiles all belong to stopped systems leave their rows
+// of y as they are.
 // `done` (may be null): a device flag; when *done != 0 the kernels return at
 // once (used by solver launch sequences).
 // ldx: 0 for row-major n x k blocks (element (i, j) at i*k + j); otherwise the
@@ -84,7 +88,8 @@ hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const vo
                        uint32_t tile_begin, uint32_t tile_end, uint32_t row_base, uint32_t empty_begin,
                        uint32_t empty_end, bool dot, const int* done, hipStream_t stream,
                        uint32_t* dot_blocks = nullptr, uint64_t ldx = 0, uint64_t ldy = 0,
-                       const double** dot_parts = nullptr);
+                       const double** dot_parts = nullptr, const uint32_t* tile_sys = nullptr,
+                       const int* sys_done = nullptr);
 
 // Same as launch_spmv over the whole matrix with events recorded before the
 // tile kernel, between the two kernels and after the fix-up kernel.

@@ -246,10 +246,14 @@ void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x,
 hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
                        uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, bool dot,
                        const int* done, hipStream_t stream, uint32_t* dot_blocks, uint64_t ldx, uint64_t ldy,
-                       const double** dot_parts) {
+                       const double** dot_parts, const uint32_t* tile_sys, const int* sys_done) {
     TileParams tp;
     FixParams fp;
     fill_params(A, S, f32, k, x, y, tile_begin, tile_end, row_base, empty_begin, empty_end, done, tp, fp, ldx, ldy);
+    if (tile_sys && sys_done && k == 1 && !dot) {
+        tp.tile_sys = tile_sys;
+        tp.sys_done = sys_done;
+    }
     uint32_t nb = 0;
     const double* parts = S.dot_blk;
     if (k == 1) {

@@ -199,6 +199,11 @@ struct TileParams {
     uint32_t empty_begin, empty_end; // empty rows written by this kernel
     int k;
     int lookahead;
+    // Batched solves (single-vector products without a dot): the system of
+    // each tile and the systems' stop flags, or null. Each system owns whole
+    // tiles, so no row crosses from one system into another.
+    const uint32_t* tile_sys;
+    const int* sys_done;
     // x and y hold k columns: element (i, j) of x at i*rs + j*xcs, of y at
     // i*rs + j*ycs (row-major n x K blocks: rs = K, xcs = ycs = 1;
     // column-major: rs = 1, xcs and ycs the leading dimensions)
@@ -228,6 +233,29 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         return; // wave-uniform
     const uint32_t t0 = p.tile_begin + c * p.tpw;
     const uint32_t t1 = min(t0 + p.tpw, p.tile_end);
+    if constexpr (KMAX == 1 && !DOT) {
+        // A chunk whose tiles all belong to systems that have stopped leaves
+        // their rows as they are (a stopped system's product is not used).
+        // It still writes its flags, as a chunk that starts a row and takes
+        // no carry, and its share of the empty rows.
+        if (p.tile_sys) {
+            const uint32_t s0 = sload(p.tile_sys + t0), s1 = sload(p.tile_sys + t1 - 1);
+            bool stopped = true;
+            for (uint32_t q = s0; q <= s1 && stopped; ++q)
+                stopped = sload(p.sys_done + q) != 0;
+            if (stopped) {
+                if (threadIdx.x % 64 == 0)
+                    p.chunk_flags[c] = kChunkStarts | kChunkRowStart;
+                if (p.lookahead) {
+                    V* __restrict__ yy = static_cast<V*>(p.y);
+                    for (uint32_t i = p.empty_begin + c * 64 + threadIdx.x % 64; i < p.empty_end;
+                         i += p.nchunks * 64)
+                        yy[static_cast<size_t>(p.empty_rows[i] - p.row_base)] = V(0);
+                }
+                return; // wave-uniform
+            }
+        }
+    }
 
     using W = typename DescWord<SIGMA>::type;
     const int k = KMAX == 1 ? 1 : p.k;
