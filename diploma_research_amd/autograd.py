@@ -19,7 +19,20 @@ from __future__ import annotations
 
 import torch
 
+from . import native as N
 from .csr5 import Csr5, conjugate_gradient_batched
+
+
+def _require_converged(results, what):
+    """Autograd callers never see the per-system CgResult, so a solve that
+    stopped at max_iters (its x or its gradient would be silently wrong)
+    raises instead."""
+    bad = [i for i, r in enumerate(results) if not r.converged]
+    if bad:
+        worst = max(results[i].relres for i in bad)
+        raise N.DimError(N.DIM_NOT_CONVERGED,
+                         f"{what}: {len(bad)} of {len(results)} systems stopped at max_iters above the tolerance "
+                         f"(first {bad[:5]}, largest relative residual {worst:.3e})")
 
 
 class CsrSolve(torch.autograd.Function):
@@ -29,8 +42,8 @@ class CsrSolve(torch.autograd.Function):
     def forward(ctx, vals, b, A: Csr5, rtol: float, max_iters: int, symmetric: bool = True):
         A.set_values(vals.detach().contiguous())
         x, res = conjugate_gradient_batched(A, b.detach().contiguous(), rtol=rtol, max_iters=max_iters)
+        _require_converged(res, "forward solve")
         ctx.A, ctx.rtol, ctx.max_iters, ctx.symmetric = A, rtol, max_iters, symmetric
-        ctx.results = res
         ctx.save_for_backward(vals.detach(), x)
         return x
 
@@ -39,8 +52,9 @@ class CsrSolve(torch.autograd.Function):
         vals, x = ctx.saved_tensors
         A = ctx.A
         A.set_values(vals.contiguous())  # the handle may have been re-valued since forward
-        lam, _ = conjugate_gradient_batched(A, gx.contiguous(), rtol=ctx.rtol, max_iters=ctx.max_iters,
-                                            transpose=not ctx.symmetric)
+        lam, res = conjugate_gradient_batched(A, gx.contiguous(), rtol=ctx.rtol, max_iters=ctx.max_iters,
+                                              transpose=not ctx.symmetric)
+        _require_converged(res, "adjoint solve")
         gvals = A.sampled_outer(lam, x, scale=-1.0) if ctx.needs_input_grad[0] else None
         gb = lam if ctx.needs_input_grad[1] else None
         return gvals, gb, None, None, None, None

@@ -11,6 +11,7 @@ from __future__ import annotations
 import argparse
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 
@@ -42,11 +43,31 @@ def _run(cmd):
     return p.returncode, out
 
 
+_INCLUDE = re.compile(r'^\s*#\s*include\s+"([^"]+)"', re.M)
+
+
+def _deps(path, seen=None):
+    """`path` and every file it includes with #include "..." (resolved against
+    the including file's folder, then csrc/ and include/), recursively."""
+    seen = set() if seen is None else seen
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        text = f.read()
+    for inc in _INCLUDE.findall(text):
+        for d in (os.path.dirname(path), CSRC, os.path.join(ROOT, "include")):
+            cand = os.path.normpath(os.path.join(d, inc))
+            if os.path.exists(cand):
+                _deps(cand, seen)
+                break
+    return seen
+
+
 def _compile(src, force=False):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     path = os.path.join(CSRC, src)
-    deps = [path] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
-    deps.append(os.path.join(ROOT, "include", "dim_gpu.h"))
+    deps = sorted(_deps(path))
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
         return src, 0, "(up to date)"
     lang = ["-x", "hip"] if src.endswith(".hip") else []
@@ -84,15 +105,18 @@ CXX = os.environ.get("CXX", "g++")
 
 
 def build_h5():
-    """libdim_h5.so (include/dim_h5.h): HDF5 matrix/vector files, host only."""
+    """libdim_h5.so (include/dim_h5.h): HDF5 matrix/vector files (host code),
+    and their loading straight to GPU matrices through libdim_gpu.so."""
     inc = os.path.join(HDF5_ROOT, "include")
     libdir = os.path.join(HDF5_ROOT, "lib")
     if not os.path.exists(os.path.join(inc, "hdf5.h")):
         print(f"HDF5 headers not found under {HDF5_ROOT}; skipping libdim_h5 and the dim CLI", file=sys.stderr)
         return None
     tmp = H5LIB + ".tmp"
+    # links libdim_gpu.so (in this folder) for the load-to-GPU calls
     cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc, "-I", os.path.join(ROOT, "include"),
-           os.path.join(CSRC, "h5io.cpp"), "-o", tmp, "-L", libdir, "-lhdf5", f"-Wl,-rpath,{libdir}"]
+           os.path.join(CSRC, "h5io.cpp"), "-o", tmp, "-L", libdir, "-lhdf5", "-L", PKG, "-ldim_gpu",
+           f"-Wl,-rpath,{libdir}", "-Wl,-rpath,$ORIGIN"]
     rc, out = _run(cmd)
     if rc != 0:
         raise RuntimeError("libdim_h5 build failed:\n" + out)

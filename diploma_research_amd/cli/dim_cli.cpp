@@ -14,6 +14,7 @@
 // The products and solves run on the GPU through libdim_gpu; files go through
 // libdim_h5.
 #include "dim/gpu/csr5.hpp"
+#include "dim/gpu/h5.hpp"
 #include "dim_h5.h"
 
 #include <hip/hip_runtime.h>
@@ -388,16 +389,37 @@ std::vector<double> vector_or(const std::optional<std::string>& file, const std:
     return v;
 }
 
+// "i/n" -> (i, n)
+std::pair<uint32_t, uint32_t> parse_part(const std::string& s) {
+    const size_t k = s.find('/');
+    if (k == std::string::npos)
+        die("--part takes i/n (part i of n, counted from 0)");
+    return {static_cast<uint32_t>(std::stoul(s.substr(0, k))), static_cast<uint32_t>(std::stoul(s.substr(k + 1)))};
+}
+
 int spmv(int argc, char** argv) {
     Args a(argc, argv, 2, {{"-n", "--num-runs"}, {"-g", "--matrix-group"}}, {"overwrite"});
     if (a.pos.size() != 1)
         die("usage: dim spmv <matrix.h5> [-n 100] [-g A] [--output-file F] [--vector-dataset Y] [--overwrite] "
-            "[--x-file F --x-dataset X] [--sigma 0] [--dtype f64|f32]");
+            "[--x-file F --x-dataset X] [--sigma 0] [--dtype f64|f32] [--part i/n]");
     const std::string input = a.pos[0];
     const bool f32 = a.get("dtype", "f64") == "f32";
+    const auto sigma = static_cast<uint32_t>(std::stoul(a.get("sigma", "0")));
     dim_h5_csr m{};
-    auto A = load_matrix(input, a.get("matrix-group", "A"), static_cast<uint32_t>(std::stoul(a.get("sigma", "0"))),
-                         f32, m);
+    // --part i/n: one rank's share (h5::load_csr5_partial and spmv<partial>,
+    // include/dim/mpi/csr5.h:40-54); y then holds its rows first .. last
+    std::optional<dim::gpu::csr5> part;
+    if (auto p = a.maybe("part")) {
+        const auto [idx, count] = parse_part(*p);
+        part.emplace(dim::io::h5::load_csr5_partial(input, a.get("matrix-group", "A"), {idx, count}, {sigma, f32}));
+        m.rows = part->last_row_idx() - part->first_row_idx() + 1;
+        m.cols = part->dimensions().cols;
+        m.nnz = part->non_zero_count();
+        info("part " + *p + ": rows " + std::to_string(part->first_row_idx()) + ".." +
+             std::to_string(part->last_row_idx()) + ", elements from " + std::to_string(part->val_offset()) +
+             " (sigma " + std::to_string(part->sigma()) + ")");
+    }
+    auto A = part ? std::move(*part) : load_matrix(input, a.get("matrix-group", "A"), sigma, f32, m);
     const uint64_t n_runs = std::stoull(a.get("num-runs", "100"));
     auto xh = vector_or(a.maybe("x-file") ? a.maybe("x-file") : std::optional<std::string>(input), a.maybe("x-dataset"),
                         m.cols, 1.0);
@@ -412,13 +434,18 @@ int spmv(int argc, char** argv) {
         HIPCHECK(hipMemcpy(x, xh.data(), xh.size() * 8, hipMemcpyHostToDevice));
     }
     auto calibrator = A.create_calibrator();
+    const bool partial = A.first_row_idx() != 0 || A.skip_tail() || a.maybe("part");
     auto run = [&] {
-        if (f32)
-            A.spmv({.x = {static_cast<const float*>(x), m.cols}, .y = {static_cast<float*>(y), m.rows},
-                    .calibrator = calibrator});
-        else
-            A.spmv({.x = {static_cast<const double*>(x), m.cols}, .y = {static_cast<double*>(y), m.rows},
-                    .calibrator = calibrator});
+        using S = dim::gpu::spmv_strategy;
+        if (f32) {
+            dim::gpu::spmv_data_t<float> d{{static_cast<const float*>(x), m.cols}, {static_cast<float*>(y), m.rows},
+                                           calibrator};
+            partial ? A.spmv<S::partial>(d) : A.spmv<S::absolute>(d);
+        } else {
+            dim::gpu::spmv_data_t<double> d{{static_cast<const double*>(x), m.cols},
+                                            {static_cast<double*>(y), m.rows}, calibrator};
+            partial ? A.spmv<S::partial>(d) : A.spmv<S::absolute>(d);
+        }
     };
     for (int i = 0; i < 5; ++i) // warm-up (main.cpp:60-62)
         run();

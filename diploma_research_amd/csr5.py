@@ -50,6 +50,31 @@ def _dtype_code(t):
     raise TypeError(f"unsupported dtype {t.dtype}")
 
 
+# the reference's message for an x that does not match the column count
+# (include/dim/mat/storage_formats/csr5.h:820-821)
+_X_COLS = "can't multiply, vector size doesn't match column count of the matrix"
+
+
+def _check_tensor(t, name, numel, dtypes=("float64",), len_msg=None):
+    """Validates a tensor before its pointer crosses the C ABI, which takes no
+    lengths for some arguments and cannot tell host from device memory: a GPU
+    tensor of one of `dtypes` (torch dtype names), contiguous, with `numel`
+    entries. Raises TypeError (not a tensor, wrong dtype, not on the GPU) or
+    ValueError (not contiguous, wrong length)."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor, got {type(t).__name__}")
+    if str(t.dtype).replace("torch.", "") not in dtypes:
+        raise TypeError(f"{name} must have dtype {' or '.join(dtypes)}, got {t.dtype}")
+    if not t.is_cuda:
+        raise TypeError(f"{name} must be on the GPU, got device {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() != numel:
+        raise ValueError(len_msg or f"{name} must have {numel} entries, got {t.numel()}")
+    return t
+
+
 @dataclass
 class Dimensions:
     """dim::mat::dimensions_t (storage_formats/base.h:10-14)."""
@@ -203,8 +228,7 @@ class Csr5:
     def set_values(self, vals, stream=None):
         """Replaces the stored values (same pattern) from a float64 GPU tensor
         in the CSR order the matrix was built from."""
-        if vals.numel() != self.non_zero_count() or not vals.is_contiguous():
-            raise ValueError("vals must be a contiguous tensor with one entry per stored element")
+        _check_tensor(vals, "vals", self.non_zero_count())
         N.check(N.lib().dim_csr5_set_values(self._h, _ptr(vals), _stream_ptr(stream)))
 
     @classmethod
@@ -212,6 +236,9 @@ class Csr5:
         """Same, with the CSR arrays already on the GPU (torch tensors:
         int32/uint32 row_ptr and col_idx, float64 vals); conversion runs on
         the GPU."""
+        _check_tensor(row_ptr, "row_ptr", rows + 1, ("int32", "uint32"))
+        _check_tensor(vals, "vals", vals.numel())
+        _check_tensor(col_idx, "col_idx", vals.numel(), ("int32", "uint32"))
         opts = N.Csr5Opts(sigma, N.DIM_CSR5_KEEP_F32 if keep_f32 else 0)
         h = C.c_void_p()
         N.check(N.lib().dim_csr5_from_csr_device(rows, cols, vals.numel(), _ptr(row_ptr), _ptr(col_idx), _ptr(vals),
@@ -275,18 +302,20 @@ class Csr5:
             x, y = data.x, data.y
         else:
             x = data
+        _check_tensor(x, "x", self.dimensions.cols, ("float64", "float32"), _X_COLS)
         if y is None:
             y = torch.empty(self.dimensions.rows, dtype=x.dtype, device=x.device)
         if x.dtype != y.dtype:
             raise TypeError("x and y must have the same dtype")
-        if not (x.is_contiguous() and y.is_contiguous()):
-            raise ValueError("x and y must be contiguous")
+        _check_tensor(y, "y", self.dimensions.rows, ("float64", "float32"))
         N.check(N.lib().dim_csr5_spmv(self._h, _dtype_code(x), _ptr(x), x.numel(), _ptr(y), y.numel(),
                                       _stream_ptr(stream)))
         return y
 
     def spmv_timed(self, x, y, stream=None):
         """y = A*x with HIP events around each kernel; returns (tiles_ms, fixup_ms)."""
+        _check_tensor(x, "x", self.dimensions.cols, ("float64", "float32"), _X_COLS)
+        _check_tensor(y, "y", self.dimensions.rows, (str(x.dtype).replace("torch.", ""),))
         ms = (C.c_float * 2)()
         N.check(N.lib().dim_csr5_spmv_timed(self._h, _dtype_code(x), _ptr(x), x.numel(), _ptr(y), y.numel(),
                                             _stream_ptr(stream), ms))
@@ -302,9 +331,11 @@ class Csr5:
         """csr5::spmv<partial> on tiles [tile_begin, tile_end): returns
         (first_row, y_block) with this block's contributions."""
         torch = _torch()
+        _check_tensor(x, "x", self.dimensions.cols, ("float64", "float32"), _X_COLS)
         first, n = self.block_rows(tile_begin, tile_end)
         if y is None:
             y = torch.empty(n, dtype=x.dtype, device=x.device)
+        _check_tensor(y, "y", n, (str(x.dtype).replace("torch.", ""),))
         N.check(N.lib().dim_csr5_spmv_block(self._h, _dtype_code(x), tile_begin, tile_end, _ptr(x), x.numel(),
                                             _ptr(y), y.numel(), _stream_ptr(stream)))
         return first, y
@@ -316,8 +347,12 @@ class Csr5:
         (k, cols) tensor) runs the column-major product, the faster layout;
         a new Y then has the same layout. Anything else is made row-major."""
         torch = _torch()
-        if X.dim() != 2 or X.shape[0] != self.dimensions.cols:
-            raise ValueError("X must have shape (cols, k)")
+        if not isinstance(X, torch.Tensor) or X.dim() != 2 or X.shape[0] != self.dimensions.cols:
+            raise ValueError("X must be a tensor of shape (cols, k)")
+        if str(X.dtype).replace("torch.", "") not in ("float64", "float32") or not X.is_cuda:
+            raise TypeError("X must be a float64 or float32 GPU tensor")
+        if Y is not None and (not isinstance(Y, torch.Tensor) or not Y.is_cuda):
+            raise TypeError("Y must be a GPU tensor")
         rows, k = self.dimensions.rows, X.shape[1]
         colmajor = k > 1 and X.stride(0) == 1 and X.stride(1) >= max(X.shape[0], 1)
         if not colmajor:
@@ -341,18 +376,21 @@ class Csr5:
     def spmv_transpose(self, x, y=None, stream=None):
         """y = A^T x."""
         torch = _torch()
-        if x.numel() != self.dimensions.rows:
-            raise ValueError("x must have rows entries")
+        _check_tensor(x, "x", self.dimensions.rows, ("float64", "float32"))
         if y is None:
             y = torch.empty(self.dimensions.cols, dtype=x.dtype, device=x.device)
+        _check_tensor(y, "y", self.dimensions.cols, (str(x.dtype).replace("torch.", ""),))
         N.check(N.lib().dim_csr5_spmv_transpose(self._h, _dtype_code(x), _ptr(x), _ptr(y), _stream_ptr(stream)))
         return y
 
     def sampled_outer(self, u, v, scale=1.0, out=None, stream=None):
         """out[e] = scale * u[row(e)] * v[col(e)] over the stored elements, CSR order."""
         torch = _torch()
+        _check_tensor(u, "u", self.dimensions.rows)
+        _check_tensor(v, "v", self.dimensions.cols)
         if out is None:
             out = torch.empty(self.non_zero_count(), dtype=torch.float64, device=u.device)
+        _check_tensor(out, "out", self.non_zero_count())
         N.check(N.lib().dim_csr5_sampled_outer(self._h, float(scale), _ptr(u), _ptr(v), _ptr(out),
                                                _stream_ptr(stream)))
         return out
@@ -390,8 +428,11 @@ def conjugate_gradient(A: Csr5, b, x=None, rtol=1e-8, max_iters=5000, precision=
     iteration inside FP64 residual replacement; needs keep_f32). Returns
     (x, CgResult); a breakdown (non-SPD matrix) raises DimError."""
     torch = _torch()
+    n = A.dimensions.rows
+    _check_tensor(b, "b", n)
     if x is None:
         x = torch.zeros_like(b)
+    _check_tensor(x, "x", n)
     hist = np.zeros(max_iters + 1) if history else None
     opts = _cg_opts(rtol, max_iters, precision, check_every, refine_delta, hist)
     res = N.CgResult()
@@ -404,9 +445,17 @@ def conjugate_gradient_multi(A: Csr5, B, X=None, rtol=1e-8, max_iters=5000, prec
                              refine_delta=0.0, stream=None):
     """k independent right-hand sides (B: rows x k) iterated with one SpMM per step."""
     torch = _torch()
-    k = B.shape[1]
+    if not isinstance(B, torch.Tensor) or B.dim() != 2:
+        raise ValueError("B must be a tensor of shape (rows, k)")
+    n, k = A.dimensions.rows, B.shape[1]
+    if B.shape[0] != n or not 1 <= k <= 16:
+        raise ValueError(f"B must have shape ({n}, k) with k in 1..16, got {tuple(B.shape)}")
+    _check_tensor(B, "B", n * k)
     if X is None:
         X = torch.zeros_like(B)
+    if not isinstance(X, torch.Tensor) or tuple(X.shape) != tuple(B.shape):
+        raise ValueError("X must have the shape of B")
+    _check_tensor(X, "X", n * k)
     opts = _cg_opts(rtol, max_iters, precision, check_every, refine_delta, None)
     res = (N.CgResult * k)()
     N.check(N.lib().dim_cg_multi(A._h, k, _ptr(B), _ptr(X), C.byref(opts), res, _stream_ptr(stream)),
@@ -421,8 +470,11 @@ def conjugate_gradient_batched(A: Csr5, B, X=None, rtol=1e-8, max_iters=5000, ch
     transpose=True the products use A^T (the adjoint solve). Returns (X,
     [CgResult per system])."""
     torch = _torch()
+    n = A.dimensions.rows  # the systems' rows together
+    _check_tensor(B, "B", n)
     if X is None:
         X = torch.zeros_like(B)
+    _check_tensor(X, "X", n)
     opts = _cg_opts(rtol, max_iters, "f64", check_every, 0.0, None, N.DIM_CG_TRANSPOSE if transpose else 0)
     S = A.n_systems
     res = (N.CgResult * S)()

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -291,14 +292,23 @@ __global__ __launch_bounds__(kVecThreads) void k_ru_fold(size_t nk, double* x, f
 
 // Reliable update, part 2: r64 = b - q64; r32 = float(r64); partials of the
 // FP32 residual's r.r (iteration scalar) and of the FP64 residual (stop test).
-__global__ __launch_bounds__(kVecThreads) void k_ru_residual(size_t n, const double* b, const double* q, float* r32,
+// A column that the FP32 recursion has marked converged stopped updating p
+// (k_cg_update_xp), so its p is stale: it restarts with p = r, which is the
+// right direction whether k_ru_state then finds it converged (p unused) or
+// resumes it. Columns still running keep p (the recurrence continues with
+// the replaced residual).
+__global__ __launch_bounds__(kVecThreads) void k_ru_residual(const CgState* st, size_t n, const double* b,
+                                                             const double* q, float* r32, float* p32,
                                                              double* rr_part, double* rt_part) {
     const size_t o = blockIdx.y * n;
+    const bool restart = st->col_done[blockIdx.y] == 1;
     double acc = 0.0, trs = 0.0;
     DIM_COL_LOOP(i) {
         const double rv = b[o + i] - q[o + i];
         const float rsv = static_cast<float>(rv);
         r32[o + i] = rsv;
+        if (restart)
+            p32[o + i] = rsv;
         acc += static_cast<double>(rsv) * static_cast<double>(rsv);
         trs += rv * rv;
     }
@@ -395,7 +405,6 @@ struct Graph {
 
 struct PinnedState {
     CgState* h = nullptr;
-    PinnedState() { (void)hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(CgState), hipHostMallocDefault); }
     ~PinnedState() {
         if (h)
             (void)hipHostFree(h);
@@ -427,56 +436,81 @@ struct Launcher {
     }
 };
 
+// The device buffers of one solve: the state, the reduction partials (nvb *
+// kMaxK each), the work vectors of the precision in use and, for k > 1, the
+// column-major copies of B and X.
+struct Work {
+    CgState* st = nullptr;
+    double *rr0 = nullptr, *rr1 = nullptr, *bb = nullptr, *rt = nullptr, *hist = nullptr;
+    double *q64 = nullptr, *r64 = nullptr, *p64 = nullptr, *bc = nullptr, *xc = nullptr;
+    float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+};
+
+dim_status alloc_work(DevBuf& buf, size_t n, int k, uint32_t nvb, bool mixed, int hist_cap, Work& w) {
+    DIM_HIP_TRY(buf.alloc(w.st, 1));
+    DIM_HIP_TRY(buf.alloc(w.q64, n * k));
+    DIM_HIP_TRY(buf.alloc(w.rr0, size_t{nvb} * kMaxK));
+    DIM_HIP_TRY(buf.alloc(w.rr1, size_t{nvb} * kMaxK));
+    DIM_HIP_TRY(buf.alloc(w.bb, size_t{nvb} * kMaxK));
+    DIM_HIP_TRY(buf.alloc(w.rt, size_t{nvb} * kMaxK));
+    if (hist_cap)
+        DIM_HIP_TRY(buf.alloc(w.hist, static_cast<size_t>(hist_cap) * k));
+    if (mixed) {
+        DIM_HIP_TRY(buf.alloc(w.r32, n * k));
+        DIM_HIP_TRY(buf.alloc(w.p32, n * k));
+        DIM_HIP_TRY(buf.alloc(w.q32, n * k));
+        DIM_HIP_TRY(buf.alloc(w.x32, n * k));
+    } else {
+        DIM_HIP_TRY(buf.alloc(w.r64, n * k));
+        DIM_HIP_TRY(buf.alloc(w.p64, n * k));
+    }
+    if (k > 1) {
+        DIM_HIP_TRY(buf.alloc(w.bc, n * k));
+        DIM_HIP_TRY(buf.alloc(w.xc, n * k));
+    }
+    return DIM_SUCCESS;
+}
+
+// fixed == nullptr: the solve allocates its buffers, captures a batch of
+// iterations in a hipGraph and polls a pinned copy of the state. Otherwise its
+// buffers are the ones `fixed` points into and it allocates nothing: no
+// device, pinned or graph memory, no events (kernels are launched directly,
+// the state is copied to the host stack, the time is host wall time).
 template<int KMAX>
 dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const dim_cg_opts& o, dim_cg_result* res,
-                 hipStream_t stream) {
+                 hipStream_t stream, const Work* fixed = nullptr) {
     const size_t n = A->f.rows;
     const bool mixed = o.precision == DIM_CG_F32_REFINED;
     if (mixed && !A->f.vals32)
         return fail(DIM_INVALID_ARGUMENT, "FP32-refined CG needs a matrix built with DIM_CSR5_KEEP_F32");
+    const bool no_alloc = fixed != nullptr;
     const uint32_t nvb = vec_blocks(n);
     const dim3 vgrid(nvb, static_cast<uint32_t>(k));
     const uint32_t check = o.check_every ? std::max<uint32_t>(2, (o.check_every + 1) & ~1u) : 16;
     const double delta = o.refine_delta > 0.0 ? o.refine_delta : 0.1;
+    const int hist_cap = o.history ? static_cast<int>(o.max_iters) + 1 : 0;
 
     DevBuf buf;
-    CgState* st = nullptr;
-    double *q64 = nullptr, *r64 = nullptr, *p64 = nullptr, *rr0 = nullptr, *rr1 = nullptr, *bb = nullptr,
-           *rt = nullptr, *hist = nullptr, *bc = nullptr, *xc = nullptr;
-    float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
-    DIM_HIP_TRY(buf.alloc(st, 1));
-    DIM_HIP_TRY(buf.alloc(q64, n * k));
-    DIM_HIP_TRY(buf.alloc(rr0, size_t{nvb} * kMaxK));
-    DIM_HIP_TRY(buf.alloc(rr1, size_t{nvb} * kMaxK));
-    DIM_HIP_TRY(buf.alloc(bb, size_t{nvb} * kMaxK));
-    DIM_HIP_TRY(buf.alloc(rt, size_t{nvb} * kMaxK));
-    const int hist_cap = o.history ? static_cast<int>(o.max_iters) + 1 : 0;
-    if (hist_cap)
-        DIM_HIP_TRY(buf.alloc(hist, static_cast<size_t>(hist_cap) * k));
-    if (mixed) {
-        DIM_HIP_TRY(buf.alloc(r32, n * k));
-        DIM_HIP_TRY(buf.alloc(p32, n * k));
-        DIM_HIP_TRY(buf.alloc(q32, n * k));
-        DIM_HIP_TRY(buf.alloc(x32, n * k));
-    } else {
-        DIM_HIP_TRY(buf.alloc(r64, n * k));
-        DIM_HIP_TRY(buf.alloc(p64, n * k));
+    Work w;
+    if (no_alloc) {
+        w = *fixed;
+    } else if (dim_status e = alloc_work(buf, n, k, nvb, mixed, hist_cap, w)) {
+        return e;
     }
-    if (k > 1) {
-        DIM_HIP_TRY(buf.alloc(bc, n * k));
-        DIM_HIP_TRY(buf.alloc(xc, n * k));
+    CgState* st = w.st;
+    double *q64 = w.q64, *r64 = w.r64, *p64 = w.p64, *rr0 = w.rr0, *rr1 = w.rr1, *bb = w.bb, *rt = w.rt,
+           *hist = w.hist;
+    float *r32 = w.r32, *p32 = w.p32, *q32 = w.q32, *x32 = w.x32;
+    const double* b = k > 1 ? w.bc : b_in;
+    double* x = k > 1 ? w.xc : x_out;
+    PinnedState pinned;
+    CgState stack_state{};
+    CgState* hs = &stack_state;
+    if (!no_alloc) {
+        DIM_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&pinned.h), sizeof(CgState), hipHostMallocDefault));
+        hs = pinned.h;
     }
-    const double* b = k > 1 ? bc : b_in;
-    double* x = k > 1 ? xc : x_out;
-    PinnedState hs;
-    if (!hs.h)
-        return fail(DIM_OUT_OF_MEMORY, "pinned host state");
 
-    // capture `check` iterations (even) once
-    Graph graph;
-    hipStream_t cap = nullptr;
-    DIM_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    bool use_graph = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
     auto enqueue_batch = [&](hipStream_t s) {
         for (uint32_t it = 0; it < check; it += 2) {
             if (mixed) {
@@ -492,25 +526,39 @@ dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const di
             }
         }
     };
-    if (use_graph) {
-        enqueue_batch(cap);
-        use_graph = hipStreamEndCapture(cap, &graph.g) == hipSuccess && graph.g &&
-                    hipGraphInstantiate(&graph.ge, graph.g, nullptr, nullptr, 0) == hipSuccess;
-        (void)hipGetLastError();
+    // capture `check` iterations (even) once
+    Graph graph;
+    bool use_graph = false;
+    if (!no_alloc) {
+        hipStream_t cap = nullptr;
+        DIM_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        use_graph = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (use_graph) {
+            enqueue_batch(cap);
+            use_graph = hipStreamEndCapture(cap, &graph.g) == hipSuccess && graph.g &&
+                        hipGraphInstantiate(&graph.ge, graph.g, nullptr, nullptr, 0) == hipSuccess;
+            (void)hipGetLastError();
+        }
+        (void)hipStreamDestroy(cap);
     }
-    (void)hipStreamDestroy(cap);
 
-    hipEvent_t e0, e1;
-    DIM_HIP_TRY(hipEventCreate(&e0));
-    DIM_HIP_TRY(hipEventCreate(&e1));
-    DIM_HIP_TRY(hipEventRecord(e0, stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const auto t_start = std::chrono::steady_clock::now();
+    if (!no_alloc) {
+        DIM_HIP_TRY(hipEventCreate(&e0));
+        DIM_HIP_TRY(hipEventCreate(&e1));
+        DIM_HIP_TRY(hipEventRecord(e0, stream));
+    }
 
     const uint32_t tgrid = 1024;
     if (k > 1) {
-        hipLaunchKernelGGL(k_rows_to_cols, dim3(tgrid), dim3(256), 0, stream, n, k, b_in, bc);
+        hipLaunchKernelGGL(k_rows_to_cols, dim3(tgrid), dim3(256), 0, stream, n, k, b_in, w.bc);
         hipLaunchKernelGGL(k_rows_to_cols, dim3(tgrid), dim3(256), 0, stream, n, k, static_cast<const double*>(x_out),
-                           xc);
+                           w.xc);
     }
+    // Every buffer is written before it is read (the state by this copy, the
+    // partials by whole-grid passes, x32 by k_zero), so memory left by an
+    // earlier solve needs no clearing.
     CgState init{};
     init.max_iters = static_cast<int>(o.max_iters);
     init.k = k;
@@ -529,41 +577,41 @@ dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const di
     DIM_HIP_TRY(hipGetLastError());
 
     uint32_t refinements = 0;
-    std::vector<double> rr_at_update(k, 0.0);
+    std::array<double, kMaxK> rr_at_update{};
     bool first = true;
     dim_status status = DIM_SUCCESS;
     for (;;) {
-        DIM_HIP_TRY(hipMemcpyAsync(hs.h, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+        DIM_HIP_TRY(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
         DIM_HIP_TRY(hipStreamSynchronize(stream));
         if (first) {
             for (int j = 0; j < k; ++j)
-                rr_at_update[j] = hs.h->rr_true[j];
+                rr_at_update[j] = hs->rr_true[j];
             first = false;
         }
-        if (mixed && hs.h->done != 2) {
+        if (mixed && hs->done != 2) {
             // residual replacement when a column's FP32 residual fell by delta
             // since the last one, when the FP32 residual says converged, or at
             // the iteration cap
-            bool need = hs.h->done == 1 || hs.h->done == 3;
+            bool need = hs->done == 1 || hs->done == 3;
             for (int j = 0; j < k && !need; ++j)
-                need = !hs.h->col_done[j] && hs.h->rr[j] < delta * delta * rr_at_update[j];
-            if (need && hs.h->iterations > 0) {
+                need = !hs->col_done[j] && hs->rr[j] < delta * delta * rr_at_update[j];
+            if (need && hs->iterations > 0) {
                 ++refinements;
                 hipLaunchKernelGGL(k_ru_fold, dim3(nvb), dim3(kVecThreads), 0, stream, n * k, x, x32);
                 DIM_HIP_TRY(launch_spmv(A->f, A->s, false, k, x, q64, 0, A->f.tiles, 0, 0, A->f.n_empty, false,
                                         nullptr, stream, nullptr, n));
-                hipLaunchKernelGGL(k_ru_residual, vgrid, dim3(kVecThreads), 0, stream, n, b,
-                                   static_cast<const double*>(q64), r32, rr0, rt);
+                hipLaunchKernelGGL(k_ru_residual, vgrid, dim3(kVecThreads), 0, stream, static_cast<const CgState*>(st),
+                                   n, b, static_cast<const double*>(q64), r32, p32, rr0, rt);
                 hipLaunchKernelGGL((k_ru_state<KMAX>), dim3(1), dim3(64), 0, stream, st, static_cast<int>(nvb), rr0,
                                    rt);
                 DIM_HIP_TRY(hipGetLastError());
-                DIM_HIP_TRY(hipMemcpyAsync(hs.h, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+                DIM_HIP_TRY(hipMemcpyAsync(hs, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
                 DIM_HIP_TRY(hipStreamSynchronize(stream));
                 for (int j = 0; j < k; ++j)
-                    rr_at_update[j] = hs.h->rr[j];
+                    rr_at_update[j] = hs->rr[j];
             }
         }
-        if (hs.h->done)
+        if (hs->done)
             break;
         if (use_graph) {
             DIM_HIP_TRY(hipGraphLaunch(graph.ge, stream));
@@ -578,18 +626,23 @@ dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const di
                             nullptr, n));
     hipLaunchKernelGGL(k_resid_parts, vgrid, dim3(kVecThreads), 0, stream, n, b, static_cast<const double*>(q64), rt);
     if (k > 1)
-        hipLaunchKernelGGL(k_cols_to_rows, dim3(tgrid), dim3(256), 0, stream, n, k, static_cast<const double*>(xc),
-                           x_out);
+        hipLaunchKernelGGL(k_cols_to_rows, dim3(tgrid), dim3(256), 0, stream, n, k,
+                           static_cast<const double*>(w.xc), x_out);
     DIM_HIP_TRY(hipGetLastError());
-    DIM_HIP_TRY(hipEventRecord(e1, stream));
-    DIM_HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    std::vector<double> parts(size_t{nvb} * k);
-    DIM_HIP_TRY(hipMemcpyAsync(parts.data(), rt, parts.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (!no_alloc) {
+        DIM_HIP_TRY(hipEventRecord(e1, stream));
+        DIM_HIP_TRY(hipEventSynchronize(e1));
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    // nvb <= kMaxVecBlocks partials per column
+    std::array<double, size_t{kMaxVecBlocks} * kMaxK> parts;
+    DIM_HIP_TRY(hipMemcpyAsync(parts.data(), rt, size_t{nvb} * k * sizeof(double), hipMemcpyDeviceToHost, stream));
     DIM_HIP_TRY(hipStreamSynchronize(stream));
+    if (no_alloc)
+        ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     if (hist_cap) {
         std::vector<double> hh(static_cast<size_t>(hist_cap) * k);
         DIM_HIP_TRY(hipMemcpy(hh.data(), hist, hh.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -598,7 +651,7 @@ dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const di
             o.history[i] = hh[static_cast<size_t>(i) * k];
     }
 
-    const CgState& S = *hs.h;
+    const CgState& S = *hs;
     for (int j = 0; j < k; ++j) {
         double tr = 0.0;
         for (uint32_t i = 0; i < nvb; ++i)

@@ -165,6 +165,29 @@ bool read_1d(hid_t loc, const char* name, std::vector<T>& out) {
     return ds.ok() && H5Dread(ds.id, Types<T>::mem(), H5S_ALL, H5S_ALL, H5P_DEFAULT, out.data()) >= 0;
 }
 
+// Elements [start, start + count) of a 1-D dataset (a hyperslab read). `per`
+// memory values make one dataset element (4 for the tile_desc arrays).
+template<typename T>
+bool read_slab(hid_t loc, const char* name, uint64_t start, uint64_t count, std::vector<T>& out,
+               hid_t mem_type = Types<T>::mem(), size_t per = 1) {
+    out.assign(count * per, T{});
+    if (!count)
+        return true;
+    Id ds(H5Dopen2(loc, name, H5P_DEFAULT), H5Dclose);
+    if (!ds.ok())
+        return false;
+    Id fs(H5Dget_space(ds.id), H5Sclose);
+    hsize_t n = 0;
+    if (H5Sget_simple_extent_ndims(fs.id) != 1 || H5Sget_simple_extent_dims(fs.id, &n, nullptr) < 0 ||
+        start + count > n)
+        return false;
+    const hsize_t st = start, ct = count;
+    if (H5Sselect_hyperslab(fs.id, H5S_SELECT_SET, &st, nullptr, &ct, nullptr) < 0)
+        return false;
+    Id ms(H5Screate_simple(1, &ct, nullptr), H5Sclose);
+    return H5Dread(ds.id, mem_type, ms.id, fs.id, H5P_DEFAULT, out.data()) >= 0;
+}
+
 Id array4_type(hid_t base) {
     const hsize_t four = 4;
     return Id(H5Tarray_create2(base, 1, &four), H5Tclose);
@@ -430,6 +453,173 @@ void dim_h5_free_csr(dim_h5_csr* m) {
     std::free(m->col_idx);
     std::free(m->vals);
     *m = dim_h5_csr{};
+}
+
+// load_csr5_partial (libs/dim/io/h5.cpp:137-173): part part_idx of part_count
+// of the group's tiles, read by hyperslabs. The partition and the row range
+// follow the reference: ceil(T / part_count) tiles per part
+// (calculate_partition, h5.cpp:95-103), the first row from the part's first
+// tile_ptr, the last row from its last tile's last descriptor column
+// (csr5_info_t::last_row_idx, csr5.h:304-320) or the matrix's last row for the
+// last part, which also takes the tail elements after the last tile. The part
+// is returned as CSR over rows [first_row, last_row] holding exactly the
+// elements [val_offset, val_end) in CSR order (its full tiles un-transposed),
+// so that its product is the part's share of y.
+dim_status dim_h5_load_csr5_partial(const char* path, const char* group, uint32_t part_idx, uint32_t part_count,
+                                    dim_h5_csr5_part* out) {
+    if (!out)
+        return fail(DIM_INVALID_ARGUMENT, "out is null");
+    *out = dim_h5_csr5_part{};
+    if (part_count == 0 || part_idx >= part_count)
+        return fail(DIM_INVALID_ARGUMENT, "part_idx must be below part_count");
+    dim_h5_group_kind kind;
+    if (dim_status st = dim_h5_group_kind_of(path, group, &kind))
+        return st;
+    if (kind != DIM_H5_CSR5)
+        return fail(DIM_INVALID_ARGUMENT, std::string("group ") + group + " holds no CSR5 matrix");
+    Id file = open_file(path, false);
+    Id g(H5Gopen2(file.id, group, H5P_DEFAULT), H5Gclose);
+    uint32_t cols = 0;
+    const int64_t T = dataset_len(g.id, "tile_desc"), nv = dataset_len(g.id, "vals"),
+                  nrp = dataset_len(g.id, "row_ptr");
+    if (T < 0 || nv < 0 || nrp < 1 || dataset_len(g.id, "col_idx") != nv || !read_u32_attr(g.id, "column_count", &cols) ||
+        static_cast<uint64_t>(T) * kRefTile > static_cast<uint64_t>(nv))
+        return fail(DIM_IO_ERROR, "malformed CSR5 group");
+    const uint32_t rows = static_cast<uint32_t>(nrp - 1);
+    const uint64_t part = (static_cast<uint64_t>(T) + part_count - 1) / part_count;
+    const uint64_t t0 = part * part_idx;
+    const bool last_part = part_idx + 1 == part_count;
+    // The reference's count = min(part, T - t0) wraps around past the last
+    // tile, and a non-last part without tiles has no last row: both refused.
+    if (t0 > static_cast<uint64_t>(T) || (!last_part && t0 == static_cast<uint64_t>(T)))
+        return fail(DIM_INVALID_ARGUMENT, "part " + std::to_string(part_idx) + " of " + std::to_string(part_count) +
+                                            " starts beyond the last tile: use fewer parts");
+    const uint64_t nt = std::min<uint64_t>(part, static_cast<uint64_t>(T) - t0);
+    std::vector<uint32_t> tile_ptr;
+    if (!read_slab(g.id, "tile_ptr", t0, nt + 1, tile_ptr))
+        return fail(DIM_IO_ERROR, "cannot read the part's tile_ptr");
+    const uint32_t first_row = tile_ptr[0] & ~kDirtyBit;
+    uint32_t last_row = rows ? rows - 1 : 0;
+    if (!last_part) {
+        // the segment of the last tile's last element: its last column's
+        // y_offset plus that column's row starts, less one
+        std::vector<uint32_t> desc, tdo_ptr, tdo;
+        Id a4 = array4_type(H5T_NATIVE_UINT32);
+        if (!read_slab(g.id, "tile_desc", t0 + nt - 1, 1, desc, a4.id, 4))
+            return fail(DIM_IO_ERROR, "cannot read the part's last tile descriptor");
+        const uint32_t col = desc[kRefOmega - 1];
+        const uint32_t y_off = col & 0x7fu, bits = (col >> 10) & 0xffffu;
+        const uint32_t rel = y_off + static_cast<uint32_t>(__builtin_popcount(bits)) - 1u;
+        const uint32_t tp = tile_ptr[nt - 1];
+        if (!(tp & kDirtyBit)) {
+            last_row = (tp & ~kDirtyBit) + rel;
+        } else {
+            if (!read_slab(g.id, "tile_desc_offset_ptr", t0 + nt - 1, 1, tdo_ptr) ||
+                !read_slab(g.id, "tile_desc_offset", uint64_t{tdo_ptr[0]} + rel, 1, tdo))
+                return fail(DIM_IO_ERROR, "cannot read the part's last tile_desc_offset");
+            last_row = (tp & ~kDirtyBit) + tdo[0];
+        }
+    }
+    if (rows == 0 || last_row < first_row || last_row >= rows)
+        return fail(DIM_IO_ERROR, "CSR5 group: tile descriptors do not match row_ptr");
+    const uint64_t v0 = t0 * kRefTile;
+    const uint64_t v1 = last_part ? static_cast<uint64_t>(nv) : v0 + nt * kRefTile;
+    const uint32_t n_rows = last_row - first_row + 1;
+    std::vector<uint32_t> rp, ci;
+    std::vector<double> vals;
+    // row_ptr entries first_row .. last_row (.. rows for the last part)
+    if (!read_slab(g.id, "row_ptr", first_row, last_part ? uint64_t{rows} + 1 - first_row : n_rows, rp) ||
+        !read_slab(g.id, "vals", v0, v1 - v0, vals) || !read_slab(g.id, "col_idx", v0, v1 - v0, ci))
+        return fail(DIM_IO_ERROR, "cannot read the part's row_ptr, vals or col_idx");
+    // un-transpose the part's full tiles (fast-track tiles, inside one row,
+    // were stored as they are, csr5.h:463-493)
+    std::vector<double> tv(kRefTile);
+    std::vector<uint32_t> tc(kRefTile);
+    for (uint64_t t = 0; t < nt; ++t) {
+        if ((tile_ptr[t] & ~kDirtyBit) == (tile_ptr[t + 1] & ~kDirtyBit))
+            continue;
+        const size_t base = static_cast<size_t>(t) * kRefTile;
+        for (uint32_t c = 0; c < kRefOmega; ++c)
+            for (uint32_t r = 0; r < kRefSigma; ++r) {
+                tv[c * kRefSigma + r] = vals[base + r * kRefOmega + c];
+                tc[c * kRefSigma + r] = ci[base + r * kRefOmega + c];
+            }
+        std::copy(tv.begin(), tv.end(), vals.begin() + base);
+        std::copy(tc.begin(), tc.end(), ci.begin() + base);
+    }
+    for (uint32_t c : ci)
+        if (c >= cols)
+            return fail(DIM_IO_ERROR, "CSR5 group: column index out of range");
+    // block row i holds the part's elements of row first_row + i
+    const uint64_t ne = v1 - v0;
+    std::vector<uint32_t> brp(size_t{n_rows} + 1);
+    for (uint32_t i = 0; i < n_rows; ++i) {
+        const uint64_t e = std::min<uint64_t>(std::max<uint64_t>(rp[i], v0), v1);
+        brp[i] = static_cast<uint32_t>(e - v0);
+    }
+    brp[n_rows] = static_cast<uint32_t>(ne);
+    for (uint32_t i = 0; i < n_rows; ++i)
+        if (brp[i + 1] < brp[i])
+            return fail(DIM_IO_ERROR, "CSR5 group: row_ptr is not non-decreasing");
+    dim_h5_csr5_part p{};
+    p.rows = rows;
+    p.cols = cols;
+    p.first_row = first_row;
+    p.last_row = last_row;
+    p.tile_begin = t0;
+    p.tile_count = nt;
+    p.val_offset = v0;
+    p.skip_tail = last_part ? 0 : 1;
+    p.block.rows = n_rows;
+    p.block.cols = cols;
+    p.block.nnz = ne;
+    p.block.row_ptr = copy_out(brp);
+    p.block.col_idx = copy_out(ci);
+    p.block.vals = copy_out(vals);
+    if (!p.block.row_ptr || !p.block.col_idx || !p.block.vals) {
+        dim_h5_free_csr(&p.block);
+        return fail(DIM_OUT_OF_MEMORY, "host allocation");
+    }
+    *out = p;
+    return DIM_SUCCESS;
+}
+
+void dim_h5_free_csr5_part(dim_h5_csr5_part* p) {
+    if (p)
+        dim_h5_free_csr(&p->block);
+}
+
+dim_status dim_csr5_load_h5(const char* path, const char* group, const dim_csr5_opts* opts, dim_csr5** out) {
+    if (!out)
+        return fail(DIM_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    dim_h5_csr m{};
+    if (dim_status st = dim_h5_read_csr(path, group, &m))
+        return st;
+    const dim_status st = dim_csr5_from_csr(m.rows, m.cols, m.nnz, m.row_ptr, m.col_idx, m.vals, opts, out);
+    dim_h5_free_csr(&m);
+    return st ? fail(st, dim_last_error()) : DIM_SUCCESS;
+}
+
+dim_status dim_csr5_load_h5_partial(const char* path, const char* group, uint32_t part_idx, uint32_t part_count,
+                                    const dim_csr5_opts* opts, dim_csr5** out, uint32_t* first_row,
+                                    uint32_t* n_rows) {
+    if (!out)
+        return fail(DIM_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    dim_h5_csr5_part part{};
+    if (dim_status st = dim_h5_load_csr5_partial(path, group, part_idx, part_count, &part))
+        return st;
+    const dim_h5_csr& b = part.block;
+    const dim_status st = dim_csr5_from_csr(b.rows, b.cols, b.nnz, b.row_ptr, b.col_idx, b.vals, opts, out);
+    if (st == DIM_SUCCESS) {
+        if (first_row)
+            *first_row = part.first_row;
+        if (n_rows)
+            *n_rows = b.rows;
+    }
+    dim_h5_free_csr5_part(&part);
+    return st ? fail(st, dim_last_error()) : DIM_SUCCESS;
 }
 
 dim_status dim_h5_write_matlab_csr(const char* path, const char* group, int append, const dim_h5_csr* m,

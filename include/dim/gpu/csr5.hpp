@@ -7,9 +7,18 @@
 //   csr5<double>::from_csr(csr)                 csr5::from_csr(csr) (host CSR in, device format out)
 //   csr5::create_calibrator()                   csr5::create_calibrator() (a token: carries live on the device)
 //   csr5::spmv<absolute>({x, y, calibrator})    csr5::spmv({x, y, calibrator}) with DEVICE spans
-//   csr5::spmv<partial>(...)                    csr5::spmv_block(tile_begin, tile_end, x, y_block)
+//   csr5::spmv<partial>(...) on a part from     the same, on a part from dim::io::h5::load_csr5_partial
+//     h5::load_csr5_partial                       (dim/gpu/h5.hpp), y holding last_row_idx() -
+//                                                 first_row_idx() + 1 entries; or csr5::spmv_block(
+//                                                 tile_begin, tile_end, x, y_block) on a whole matrix
 //   csr5::tail_partition_start(), first/last_row_idx(), non_zero_count(), dimensions
 //   conjugate_gradient(A, b, x)                 dim::gpu::conjugate_gradient(A, b, x, opts)
+//
+// dim::mat::csr5<V, Sigma, Omega, U, Storage>, dim::mat::csr<V> and
+// dim::mat::dimensions_t name the same types, so that code spelled with the
+// reference's namespace compiles unchanged (the GPU tile is omega = 64 with
+// sigma from the autotuner, so the Sigma/Omega/U/Storage arguments are
+// accepted and not used; V must be double).
 //
 // Errors are thrown as in the reference: std::invalid_argument for bad
 // arguments (csr5.h:820-821), std::runtime_error for everything else. Header
@@ -19,6 +28,7 @@
 
 #include "../../dim_gpu.h"
 
+#include <cstddef>
 #include <cstdint>
 #include <memory>
 #include <span>
@@ -76,6 +86,27 @@ struct csr5_options {
 class csr5 {
   public:
     using value_type = double;
+    using spmv_strategy = ::dim::gpu::spmv_strategy;
+    using calibrator_type = calibrator_t;
+    using spmv_data_t = ::dim::gpu::spmv_data_t<double>;
+
+    // One rank's share loaded by h5::load_csr5_partial: the rows of the
+    // whole matrix that its products write, and where its elements start.
+    struct part_t {
+        uint32_t first_row = 0, n_rows = 0;
+        uint64_t val_offset = 0;
+        bool skip_tail = false;
+    };
+
+    // takes ownership of a handle from the C ABI; `part` marks a loaded share
+    static csr5 adopt(dim_csr5* h, const part_t* part = nullptr) {
+        csr5 m{h};
+        if (part) {
+            m.part_ = *part;
+            m.is_part_ = true;
+        }
+        return m;
+    }
 
     // csr5::from_csr (csr5.h:791-803)
     static csr5 from_csr(const csr& m, const csr5_options& o = {}) {
@@ -95,13 +126,20 @@ class csr5 {
     calibrator_t create_calibrator() const { return {}; }
 
     // y = A x over device spans; x.size() must equal the column count (the
-    // reference throws invalid_argument otherwise, csr5.h:820-821)
+    // reference throws invalid_argument otherwise, csr5.h:820-821). The
+    // partial strategy is for a part from h5::load_csr5_partial: y then holds
+    // its rows first_row_idx() .. last_row_idx() (csr5.h:585-594).
     template<spmv_strategy S = spmv_strategy::absolute>
-    void spmv(spmv_data_t<double> d, void* stream = nullptr) const {
-        static_assert(S == spmv_strategy::absolute, "use spmv_block for a row block (partial strategy)");
+    void spmv(::dim::gpu::spmv_data_t<double> d, void* stream = nullptr) const {
+        if (S == spmv_strategy::partial && !is_part_)
+            throw std::invalid_argument{"spmv<partial> needs a part loaded by h5::load_csr5_partial "
+                                        "(or spmv_block on a whole matrix)"};
         check(dim_csr5_spmv(h_.get(), DIM_F64, d.x.data(), d.x.size(), d.y.data(), d.y.size(), stream));
     }
-    void spmv(spmv_data_t<float> d, void* stream = nullptr) const {
+    template<spmv_strategy S = spmv_strategy::absolute>
+    void spmv(::dim::gpu::spmv_data_t<float> d, void* stream = nullptr) const {
+        if (S == spmv_strategy::partial && !is_part_)
+            throw std::invalid_argument{"spmv<partial> needs a part loaded by h5::load_csr5_partial"};
         check(dim_csr5_spmv(h_.get(), DIM_F32, d.x.data(), d.x.size(), d.y.data(), d.y.size(), stream));
     }
 
@@ -138,6 +176,14 @@ class csr5 {
     uint64_t non_zero_count() const { return info().nnz; }
     uint32_t tile_count() const { return info().tile_count; }
     uint32_t tail_partition_start() const { return info().tail_partition_start; }
+    // the first and last row a product writes (csr5.h:849-855): the whole
+    // matrix, or a loaded part's rows
+    uint32_t first_row_idx() const { return is_part_ ? part_.first_row : 0u; }
+    uint32_t last_row_idx() const {
+        return is_part_ ? part_.first_row + part_.n_rows - 1 : dimensions().rows - 1;
+    }
+    uint64_t val_offset() const { return is_part_ ? part_.val_offset : 0u; }
+    bool skip_tail() const { return is_part_ && part_.skip_tail; }
     uint32_t sigma() const { return info().sigma; }
     static constexpr uint32_t omega() { return 64; }
     dim_csr5_info info() const {
@@ -154,6 +200,8 @@ class csr5 {
     };
     explicit csr5(dim_csr5* h) : h_(h) {}
     std::unique_ptr<dim_csr5, deleter> h_;
+    part_t part_{};
+    bool is_part_ = false;
 };
 
 struct cg_options {
@@ -179,5 +227,18 @@ inline dim_cg_result conjugate_gradient(const csr5& A, const double* b, double* 
 }
 
 } // namespace dim::gpu
+
+// The reference's spelling of these types (csr5.h:254-260, csr.h:19-23,
+// base.h:10-14).
+namespace dim::mat {
+using ::dim::gpu::dimensions_t;
+template<typename V = double, std::size_t Sigma = 16, std::size_t Omega = 4, typename U = std::uint32_t,
+         template<typename> class Storage = std::vector>
+using csr5 = ::dim::gpu::csr5;
+template<typename V = double, template<typename> class Storage = std::vector>
+using csr = ::dim::gpu::csr;
+template<typename V = double>
+using calibrator_t = ::dim::gpu::calibrator_t;
+} // namespace dim::mat
 
 #endif // DIM_GPU_CSR5_HPP

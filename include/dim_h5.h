@@ -19,7 +19,8 @@
  *
  * Errors are returned as dim_status codes with a message in
  * dim_h5_last_error() (thread-local); the reference throws instead
- * (include/dim/io/h5/err.h:8-16). */
+ * (include/dim/io/h5/err.h:8-16). The two dim_csr5_load_h5* calls also build
+ * the GPU matrix, so this library links libdim_gpu.so. */
 #ifndef DIM_H5_H
 #define DIM_H5_H
 
@@ -79,6 +80,50 @@ dim_status dim_h5_group_kind_of(const char* path, const char* group, dim_h5_grou
  * inverse of the tile transpose, or read_matlab_compatible). */
 dim_status dim_h5_read_csr(const char* path, const char* group, dim_h5_csr* out);
 void dim_h5_free_csr(dim_h5_csr* m);
+
+/* One rank's share of a CSR5 group: dim::io::h5::load_csr5_partial
+ * (libs/dim/io/h5.cpp:137-173) of part part_idx of part_count, read by
+ * hyperslabs. The tiles are split as the reference splits them
+ * (ceil(T / part_count) per part, h5.cpp:95-103); the part covers the rows
+ * first_row_idx() .. last_row_idx() (csr5.h:303-320, 850-855) and the stored
+ * elements [val_offset, val_offset + block.nnz) (the last part also takes the
+ * tail elements after the last tile; skip_tail marks the others, csr5.h:457-458,
+ * 756-782). `block` holds those elements as CSR with block.rows =
+ * last_row - first_row + 1 rows of the global column count, row i being the
+ * part's elements of row first_row + i in CSR order: its product
+ * (dim_csr5_from_csr, dim_csr5_spmv) is the part's share of y, the first and
+ * last rows being partial sums when the part starts or ends inside them, as
+ * with spmv<partial>. Parts that would start past the last tile (the
+ * reference wraps around there) are DIM_INVALID_ARGUMENT. Release with
+ * dim_h5_free_csr5_part. */
+typedef struct dim_h5_csr5_part {
+    uint32_t rows, cols;   /* the whole matrix */
+    uint32_t first_row;    /* first_row_idx() */
+    uint32_t last_row;     /* last_row_idx() */
+    uint64_t tile_begin;   /* the part's first reference tile (omega = 4, sigma = 16) */
+    uint64_t tile_count;   /* its full tiles */
+    uint64_t val_offset;   /* index of its first element in the group's vals */
+    int32_t skip_tail;     /* 1 unless this is the last part */
+    dim_h5_csr block;
+} dim_h5_csr5_part;
+dim_status dim_h5_load_csr5_partial(const char* path, const char* group, uint32_t part_idx, uint32_t part_count,
+                                    dim_h5_csr5_part* out);
+void dim_h5_free_csr5_part(dim_h5_csr5_part* p);
+
+/* h5::load_csr5 (libs/dim/io/h5.cpp:57-74, 174-177) straight to a GPU matrix:
+ * dim_h5_read_csr of either group layout, then dim_csr5_from_csr with `opts`
+ * (may be NULL: sigma 16), on the current device. The single call that the
+ * reference's benchmarks make (benchmarks/csr5/dim/main.cpp:51). */
+dim_status dim_csr5_load_h5(const char* path, const char* group, const dim_csr5_opts* opts, dim_csr5** out);
+
+/* load_csr5_partial straight to a GPU matrix holding one rank's share: the
+ * part's block (see dim_h5_load_csr5_partial) built with dim_csr5_from_csr.
+ * Its products (dim_csr5_spmv with y of *n_rows entries) give y rows
+ * *first_row .. *first_row + *n_rows - 1, as spmv<partial> of the loaded part
+ * does (include/dim/mpi/csr5.h:51-54). first_row / n_rows may be NULL. */
+dim_status dim_csr5_load_h5_partial(const char* path, const char* group, uint32_t part_idx, uint32_t part_count,
+                                    const dim_csr5_opts* opts, dim_csr5** out, uint32_t* first_row,
+                                    uint32_t* n_rows);
 
 /* Writes `m` as a MATLAB CSR group (write_matlab_compatible) or as a
  * reference CSR5 group (csr5::from_csr + h5::store). The file is created (or

@@ -326,6 +326,12 @@ def _rh5():
         L.dimref_h5_csr5_free.argtypes = [C.c_void_p]
         L.dimref_h5_csr5_sizes.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
         L.dimref_h5_csr5_copy.argtypes = [C.c_void_p, _u32p, _u32p, _u32p, _u32p, _u32p, _f64p, _u32p]
+        L.dimref_h5_load_csr5_partial.argtypes = [C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64]
+        L.dimref_h5_load_csr5_partial.restype = C.c_void_p
+        L.dimref_h5_part_info.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+        L.dimref_h5_part_copy.argtypes = [C.c_void_p, _u32p, _u32p, _f64p, _u32p]
+        L.dimref_h5_part_spmv.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p]
+        L.dimref_h5_part_spmv.restype = C.c_int
         L.dimref_h5_read_matlab.argtypes = [C.c_char_p, C.c_char_p]
         L.dimref_h5_read_matlab.restype = C.c_void_p
         L.dimref_h5_csr_free.argtypes = [C.c_void_p]
@@ -371,6 +377,34 @@ def ref_h5_load_csr5(path, group="A") -> dict:
                    tile_desc_offset=np.zeros(ntdo, np.uint32), vals=np.zeros(nnz), col_idx=np.zeros(nnz, np.uint32))
         L.dimref_h5_csr5_copy(h, out["row_ptr"], out["tile_ptr"], out["tile_desc"], out["tile_desc_offset_ptr"],
                               out["tile_desc_offset"], out["vals"], out["col_idx"])
+        return out
+    finally:
+        L.dimref_h5_csr5_free(h)
+
+
+def ref_h5_load_csr5_partial(path, group, idx, count, x=None, nthreads=4) -> dict:
+    """h5::load_csr5_partial (h5.cpp:137-173) of part `idx` of `count`: the
+    part's global dimensions, first_row_idx() / last_row_idx(), val_offset,
+    skip_tail, tile count and loaded slabs; with x, also y_block from the
+    reference's csr5::spmv<partial> on that part (zero-filled y, fresh
+    calibrator)."""
+    L = _rh5()
+    h = L.dimref_h5_load_csr5_partial(os.fspath(path).encode(), group.encode(), int(idx), int(count))
+    if not h:
+        raise RuntimeError("reference load_csr5_partial failed")
+    try:
+        info = np.zeros(9, np.uint64)
+        L.dimref_h5_part_info(h, info)
+        rows, cols, first, last, val_offset, skip_tail, T, nv, nrp = (int(i) for i in info)
+        out = dict(rows=rows, cols=cols, first_row=first, last_row=last, val_offset=val_offset,
+                   skip_tail=bool(skip_tail), tile_count=T, row_ptr=np.zeros(nrp, np.uint32),
+                   tile_ptr=np.zeros(T + 1, np.uint32), vals=np.zeros(nv), col_idx=np.zeros(nv, np.uint32))
+        L.dimref_h5_part_copy(h, out["row_ptr"], out["tile_ptr"], out["vals"], out["col_idx"])
+        if x is not None:
+            y = np.zeros(last - first + 1)
+            if L.dimref_h5_part_spmv(h, int(nthreads), np.ascontiguousarray(x, np.float64), y):
+                raise RuntimeError("reference spmv<partial> failed")
+            out["y_block"] = y
         return out
     finally:
         L.dimref_h5_csr5_free(h)

@@ -15,6 +15,9 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <span>
+
+#include <omp.h>
 
 namespace h5 = dim::io::h5;
 using csr5_t = dim::mat::csr5<double>;
@@ -97,6 +100,62 @@ void dimref_h5_csr5_copy(void* h, uint32_t* row_ptr, uint32_t* tile_ptr, uint32_
     std::copy(info.tile_desc_offset.begin(), info.tile_desc_offset.end(), tile_desc_offset);
     std::copy(m.vals.begin(), m.vals.end(), vals);
     std::copy(m.col_idx.begin(), m.col_idx.end(), col_idx);
+}
+
+// h5::load_csr5_partial (h5.cpp:137-173): part `idx` of `count` of the group's
+// tiles, as one rank of the distributed CG loads it; null on failure.
+void* dimref_h5_load_csr5_partial(const char* path, const char* group, uint64_t idx, uint64_t count) {
+    try {
+        const auto file = h5::file_t::open(path, H5F_ACC_RDONLY);
+        return new csr5_t(h5::load_csr5_partial(file.open_group(group), h5::partial_identifier_t{idx, count}));
+    } catch (const std::exception&) {
+        return nullptr;
+    }
+}
+
+// rows, cols (global), first_row_idx(), last_row_idx() (csr5.h:845-855),
+// val_offset, skip_tail, tile_count, loaded elements, loaded row_ptr entries
+void dimref_h5_part_info(void* h, uint64_t* out9) {
+    const auto& m = *static_cast<csr5_t*>(h);
+    out9[0] = m.dimensions.rows;
+    out9[1] = m.dimensions.cols;
+    out9[2] = m.first_row_idx();
+    out9[3] = m.last_row_idx();
+    out9[4] = m.val_offset;
+    out9[5] = m.skip_tail ? 1 : 0;
+    out9[6] = m.csr5_info.tile_count;
+    out9[7] = m.vals.size();
+    out9[8] = m.row_ptr.size();
+}
+
+// the loaded slabs: row_ptr (absolute element indices), tile_ptr (raw words),
+// vals/col_idx (as stored, tile-transposed)
+void dimref_h5_part_copy(void* h, uint32_t* row_ptr, uint32_t* tile_ptr, double* vals, uint32_t* col_idx) {
+    const auto& m = *static_cast<csr5_t*>(h);
+    std::copy(m.row_ptr.begin(), m.row_ptr.end(), row_ptr);
+    for (size_t i = 0; i < m.csr5_info.tile_ptr.size(); ++i)
+        tile_ptr[i] = m.csr5_info.tile_ptr[i].raw;
+    std::copy(m.vals.begin(), m.vals.end(), vals);
+    std::copy(m.col_idx.begin(), m.col_idx.end(), col_idx);
+}
+
+// csr5::spmv<spmv_strategy::partial> (csr5.h:818-829) of a loaded part with
+// `nthreads` OpenMP threads: y_block (last_row_idx() - first_row_idx() + 1
+// entries) is zero-filled and the calibrator is fresh, as the reference
+// requires (its tile loop adds into y). x has the global column count.
+int dimref_h5_part_spmv(void* h, int nthreads, const double* x, double* y_block) {
+    try {
+        const auto& m = *static_cast<csr5_t*>(h);
+        omp_set_num_threads(nthreads);
+        const size_t n = m.last_row_idx() - m.first_row_idx() + 1;
+        std::fill(y_block, y_block + n, 0.0);
+        auto calibrator = m.create_calibrator();
+        m.spmv<csr5_t::spmv_strategy::partial>(
+          csr5_t::spmv_data_t{std::span<const double>(x, m.dimensions.cols), std::span<double>(y_block, n), calibrator});
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
 }
 
 // read_matlab_compatible (h5.h:87-97); null on failure.

@@ -61,3 +61,35 @@ def test_null_handle_rejected():
     L = N.lib()
     assert L.dim_csr5_spmv(None, 0, None, 0, None, 0, None) == N.DIM_INVALID_ARGUMENT
     assert L.dim_cg(None, None, None, None, None, None) in (N.DIM_INVALID_ARGUMENT,)
+
+
+def test_cpp_headers_compile_with_the_reference_spelling(tmp_path):
+    """Host code spelled with the reference's names (dim::mat::csr5<double>,
+    csr5::spmv_strategy::partial, dim::io::h5::load_csr5_partial) compiles
+    against include/dim/gpu/*.hpp (syntax and types only)."""
+    import shutil
+    import subprocess
+
+    cxx = shutil.which("g++")
+    if not cxx:
+        pytest.skip("no g++")
+    src = tmp_path / "use.cpp"
+    src.write_text(r'''
+#include <dim/gpu/csr5.hpp>
+#include <dim/gpu/h5.hpp>
+using matrix_type = dim::mat::csr5<double>;
+void rank_share(const double* x, double* y) {
+    auto A = dim::io::h5::load_csr5_partial("m.csr5.h5", "A", dim::io::h5::partial_identifier_t{1, 8});
+    auto calibrator = A.create_calibrator();
+    const size_t n = A.last_row_idx() - A.first_row_idx() + 1;
+    A.spmv<matrix_type::spmv_strategy::partial>(
+        matrix_type::spmv_data_t{{x, A.dimensions().cols}, {y, n}, calibrator});
+    dim::mat::csr<double> c{dim::mat::dimensions_t{2, 2}, {1.0, 1.0}, {0, 1, 2}, {0, 1}};
+    auto B = matrix_type::from_csr(c);
+    auto W = dim::io::h5::load_csr5("m.csr5.h5", "A");
+    (void)B; (void)W;
+}
+''')
+    p = subprocess.run([cxx, "-std=c++20", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert p.returncode == 0, p.stdout

@@ -224,3 +224,52 @@ def test_adjoint_gradient_wrt_b_by_finite_differences_on_gpu(cuda):
             xs_pm.append(xx.cpu().numpy())
         fd = 0.5 * float((xs_pm[0] - xs_pm[1]) @ (xs_pm[0] + xs_pm[1])) / (2 * h)
         assert abs(fd - g[idx]) <= 1e-6 * max(abs(g[idx]), 1e-3), (idx, fd, g[idx])
+
+
+def test_autograd_raises_when_a_solve_stops_at_max_iters(cuda):
+    """The autograd path gives its caller no CgResult, so a forward or
+    adjoint solve that stops at max_iters raises DimError instead of
+    returning a wrong x or wrong gradients."""
+    import torch
+
+    from diploma_research_amd import DimError
+    from diploma_research_amd.autograd import solve
+
+    systems = _systems([30, 41], 0.5, seed=41)
+    A = Csr5.from_csr_batched(systems, sigma=8)
+    vals = torch.from_numpy(np.concatenate([v for *_, v in systems])).cuda().requires_grad_(True)
+    b = torch.from_numpy(np.random.default_rng(3).uniform(-1, 1, 900 + 1681)).cuda().requires_grad_(True)
+    with pytest.raises(DimError, match="forward solve"):
+        solve(A, vals, b, rtol=1e-12, max_iters=3)
+    x = solve(A, vals, b, rtol=1e-12, max_iters=20000)
+    # the custom Function's backward node is its ctx: cap the adjoint solve
+    x.grad_fn.max_iters = 3
+    with pytest.raises(DimError, match="adjoint solve"):
+        (0.5 * (x * x).sum()).backward()
+
+
+def test_batched_chunks_mixing_stopped_and_running_systems(cuda):
+    """Chunks of several tiles whose tiles belong to systems that stop early
+    (tiny ones) and to one slow, large system placed between them: the
+    product skips a chunk only when all of its systems have stopped, and the
+    carry fix-up may then leave stale sums in stopped systems' rows of q,
+    which no later pass reads. Every system, the large one above all, must
+    still match the oracle's CG (iterations within 2, solution within the
+    tolerance's reach)."""
+    rng = np.random.default_rng(77)
+    sides = [int(s) for s in rng.integers(4, 12, size=60)]
+    sides = sides[:30] + [700] + sides[30:]
+    systems = _systems(sides, 0.5, seed=700)
+    A = Csr5.from_csr_batched(systems, sigma=4)
+    total_tiles = A.tile_count
+    assert total_tiles > 8192, total_tiles  # more tiles than resident waves: chunks of several tiles
+    bs = [rng.uniform(-1, 1, n) for n, *_ in systems]
+    X, res = conjugate_gradient_batched(A, to_dev(np.concatenate(bs)), rtol=1e-10, max_iters=20000)
+    Xh = X.cpu().numpy()
+    off = 0
+    for (n, rp, ci, v), b, r in zip(systems, bs, res):
+        oc = O.Csr5(n, n, rp, ci, v).cg(b, rtol=1e-10, maxit=20000)
+        assert r.converged and abs(r.iterations - oc["iterations"]) <= 2, (n, r.iterations, oc["iterations"])
+        err = np.linalg.norm(Xh[off:off + n] - oc["x"]) / max(np.linalg.norm(oc["x"]), 1e-300)
+        assert err < 1e-7, (n, err)
+        off += n

@@ -191,3 +191,54 @@ def test_cg_fp32_refined_reaches_fp64_accuracy(cuda, k):
         assert res[j].iterations <= 1.5 * oc["iterations"] + 20, (res[j].iterations, oc["iterations"])
         err = np.linalg.norm(X[:, j].cpu().numpy() - oc["x"]) / np.linalg.norm(oc["x"])
         assert err < 1e-6, err
+
+
+def test_cg_fp32_refined_k8_varcoef_lognormal_64(cuda):
+    """The mixed-precision benchmark's shape at 64^3: FP32-refined CG with
+    k = 8 right-hand sides (the FP32 tile kernel for 8 sides, the 8-column
+    vector kernels and FP64 residual replacement) on the lognormal(0,1)
+    variable-coefficient operator, with the autotuned format the benchmark
+    uses. Every column against the oracle's FP64 CG: converged, FP64 true
+    relative residual <= 1.01e-10, iterations <= 1.5x the oracle's + 20, and
+    the solution's error against the oracle's x stated and bounded."""
+    side, k = 64, 8
+    rp, ci, v = matgen.varcoef((side, side, side), 1.0, seed=4)
+    n = side ** 3
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=0, keep_f32=True)
+    B = np.random.default_rng(5).uniform(-1, 1, (n, k))
+    X, res = conjugate_gradient_multi(A, to_dev(B), rtol=1e-10, max_iters=20000, precision="f32-refined")
+    Xh = X.cpu().numpy()
+    O64 = O.Csr5(n, n, rp, ci, v)
+    errs = []
+    for j in range(k):
+        oc = O64.cg(B[:, j].copy(), rtol=1e-10, maxit=20000, nthreads=16)
+        assert oc["converged"]
+        assert res[j].converged, (j, res[j])
+        assert res[j].relres_true <= 1.01e-10, (j, res[j].relres_true)
+        assert res[j].iterations <= 1.5 * oc["iterations"] + 20, (j, res[j].iterations, oc["iterations"])
+        errs.append(np.linalg.norm(Xh[:, j] - oc["x"]) / np.linalg.norm(oc["x"]))
+    print(f"sigma {A.sigma}, codes {A.info.column_codes}, refinements {res[0].refinements}, "
+          f"iterations {[r.iterations for r in res]}, rel err vs oracle x {max(errs):.2e}")
+    assert max(errs) < 1e-6, errs
+
+
+def test_cg_fp32_refined_resumes_with_restarted_direction(cuda):
+    """Residual replacement only when the FP32 recursion says converged (a
+    negligible refine_delta), on a high-contrast operator (lognormal(0,2)):
+    the FP64 residual is then still above the tolerance, so columns resume,
+    each from p = r (the direction a converged column stopped updating is
+    stale). Every column converges to the FP64 tolerance within twice the
+    oracle's iterations + 50, after more than one replacement."""
+    side, k = 40, 4
+    rp, ci, v = matgen.varcoef((side, side, side), 2.0, seed=21)
+    n = side ** 3
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=7, keep_f32=True)
+    B = np.random.default_rng(22).uniform(-1, 1, (n, k))
+    X, res = conjugate_gradient_multi(A, to_dev(B), rtol=1e-10, max_iters=40000, precision="f32-refined",
+                                      refine_delta=1e-30)
+    O64 = O.Csr5(n, n, rp, ci, v)
+    for j in range(k):
+        oc = O64.cg(B[:, j].copy(), rtol=1e-10, maxit=40000, nthreads=16)
+        assert res[j].converged and res[j].relres_true <= 1.01e-10, (j, res[j])
+        assert res[j].iterations <= 2 * oc["iterations"] + 50, (j, res[j].iterations, oc["iterations"])
+    assert res[0].refinements >= 2, res[0]

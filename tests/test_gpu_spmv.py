@@ -293,6 +293,32 @@ def test_powerlaw_4m_full_size(cuda):
     assert_spmv_close(y, O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
 
 
+@pytest.mark.parametrize("form", ["sigma16", "autotuned"])
+def test_powerlaw_4m_full_size_fp32(cuda, form):
+    """The power-law matrix at n = 4,000,000 in FP32 (the benchmark's FP32
+    leg: values and vectors in FP32) against the FP64 oracle, in the plain
+    form and in the autotuned one (the column-sliced form when the tuner
+    picks it): every row within the FP32 rounding bound (len_i + 4) * u32 *
+    sum_j |a_ij x_j| (input rounding of A and x included), the normwise error
+    stated."""
+    import torch
+
+    n = 4_000_000
+    rp, ci, v = matgen.powerlaw(n, seed=2024)
+    x = np.random.default_rng(7).uniform(-1, 1, n)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=16 if form == "sigma16" else 0, keep_f32=True)
+    y32 = A.spmv(to_dev(x, torch.float32)).double().cpu().numpy()
+    ref = O.csr_spmv(rp, ci, v, x)
+    u32 = np.finfo(np.float32).eps / 2
+    lens = np.diff(rp.astype(np.int64))
+    absum = np.bincount(np.repeat(np.arange(n), lens), weights=np.abs(v * x[ci]), minlength=n)
+    bad = np.abs(y32 - ref) > (lens + 4) * u32 * absum + 1e-30
+    assert not bad.any(), np.nonzero(bad)[0][:5]
+    rel = np.linalg.norm(y32 - ref) / np.linalg.norm(ref)
+    print(f"{form}: sigma {A.sigma}, slices {A.info.xcd_slices}, FP32 normwise error vs FP64 oracle {rel:.2e}")
+    assert rel < 1e-6
+
+
 @pytest.mark.parametrize("sigma", [4, 5, 6, 7, 8])
 def test_column_codes_equal_plain_columns(cuda, sigma):
     """Stencil operators keep column codes (their elements lie on 7 or 5

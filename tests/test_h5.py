@@ -153,7 +153,8 @@ def test_library_exports_every_declared_symbol():
     import re
 
     hdr = open(os.path.join(os.path.dirname(GOLDEN), "..", "include", "dim_h5.h")).read()
-    declared = set(re.findall(r"\b(dim_h5_\w+)\s*\(", hdr))
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    declared = set(re.findall(r"\b(dim_h5_\w+|dim_csr5_load_h5\w*)\s*\(", hdr))
     assert declared == set(h5.EXPORTS)
     L = h5.lib()
     for name in declared:
