@@ -32,6 +32,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace dimgpu {
@@ -689,6 +691,73 @@ dim_status run(dim_csr5* A, int k, const double* b, double* x, const dim_cg_opts
     return solve<16>(A, k, b, x, *opts, res, s);
 }
 
+// ---- allocation-free solves (dim_cg_noalloc) -------------------------------
+// One right-hand side of at most kNoAllocMaxRows rows. Layout of the array:
+// the state, four partial arrays (rr0, rr1, bb, rt), then q64 and either r64,
+// p64 (FP64) or r32, p32, q32, x32 (FP32-refined): 24 bytes per row either
+// way, each piece starting on a 256-byte boundary.
+constexpr size_t kNoAllocMaxRows = DIM_CG_NOALLOC_MAX_ROWS;
+constexpr size_t kNoAllocAlign = 256;
+constexpr size_t kNoAllocStateBytes = 1024;
+constexpr size_t kNoAllocPartBytes = size_t{kMaxVecBlocks} * kMaxK * sizeof(double);
+constexpr size_t kNoAllocBytes = kNoAllocStateBytes + 4 * kNoAllocPartBytes + 24 * kNoAllocMaxRows + 8 * kNoAllocAlign;
+static_assert(sizeof(CgState) <= kNoAllocStateBytes, "CgState must fit its slot");
+static_assert(kNoAllocBytes == DIM_CG_NOALLOC_BYTES, "the header states the array's size");
+
+// Each GPU holds its own copy from the library's load until the process
+// ends. Host code must not use this name as an address: the address on each
+// device comes from hipGetSymbolAddress (scratch_base).
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char* g_scratch_base[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+dim_status scratch_base(char** out) {
+    int dev = 0;
+    DIM_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(DIM_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> g(g_scratch_mutex);
+    if (!g_scratch_base[dev]) {
+        void* p = nullptr;
+        DIM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_base[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_base[dev];
+    return DIM_SUCCESS;
+}
+
+// Carves the buffers of an n-row solve out of the array at `base`.
+Work carve_noalloc(char* base, size_t n, bool mixed) {
+    auto up = [](size_t b) { return (b + kNoAllocAlign - 1) / kNoAllocAlign * kNoAllocAlign; };
+    Work w;
+    size_t at = 0;
+    w.st = reinterpret_cast<CgState*>(base);
+    at += kNoAllocStateBytes;
+    double** parts[4] = {&w.rr0, &w.rr1, &w.bb, &w.rt};
+    for (double** pp : parts) {
+        *pp = reinterpret_cast<double*>(base + at);
+        at += kNoAllocPartBytes;
+    }
+    w.q64 = reinterpret_cast<double*>(base + at);
+    at += up(n * sizeof(double));
+    if (mixed) {
+        float** vs[4] = {&w.r32, &w.p32, &w.q32, &w.x32};
+        for (float** pv : vs) {
+            *pv = reinterpret_cast<float*>(base + at);
+            at += up(n * sizeof(float));
+        }
+    } else {
+        w.r64 = reinterpret_cast<double*>(base + at);
+        at += up(n * sizeof(double));
+        w.p64 = reinterpret_cast<double*>(base + at);
+    }
+    return w;
+}
+
 dim_status check_cg_args(const dim_csr5* A, uint32_t k, const void* b, const void* x, const dim_cg_opts* opts,
                          const dim_cg_result* res) {
     if (!A)
@@ -724,6 +793,27 @@ dim_status dim_cg(const dim_csr5* A, const double* b, double* x, const dim_cg_op
     if (dim_status st = check_cg_args(A, 1, b, x, opts, res))
         return st;
     return run(const_cast<dim_csr5*>(A), 1, b, x, opts, res, static_cast<hipStream_t>(stream));
+}
+
+dim_status dim_cg_noalloc(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts,
+                          dim_cg_result* res, void* stream) {
+    if (dim_status st = check_cg_args(A, 1, b, x, opts, res))
+        return st;
+    if (opts->history)
+        return fail(DIM_INVALID_ARGUMENT, "dim_cg_noalloc records no history (opts->history must be NULL)");
+    if (A->f.rows > kNoAllocMaxRows)
+        return fail(DIM_INVALID_ARGUMENT, "dim_cg_noalloc solves at most " + std::to_string(kNoAllocMaxRows) +
+                                            " rows (the matrix has " + std::to_string(A->f.rows) + ")");
+    int dev = 0;
+    DIM_HIP_TRY(hipGetDevice(&dev));
+    if (dev != A->device)
+        return fail(DIM_INVALID_ARGUMENT, "the matrix lives on device " + std::to_string(A->device) +
+                                            ", which must be the current device");
+    char* base = nullptr;
+    if (dim_status st = scratch_base(&base))
+        return st;
+    const Work w = carve_noalloc(base, A->f.rows, opts->precision == DIM_CG_F32_REFINED);
+    return solve<1>(const_cast<dim_csr5*>(A), 1, b, x, *opts, res, static_cast<hipStream_t>(stream), &w);
 }
 
 dim_status dim_cg_multi(const dim_csr5* A, uint32_t k, const double* B, double* X, const dim_cg_opts* opts,

@@ -28,13 +28,15 @@ DIM_CSR5_PLAIN_COLUMNS = 4
 DIM_CG_F64 = 0
 DIM_CG_F32_REFINED = 1
 DIM_CG_TRANSPOSE = 1
+DIM_CG_NOALLOC_MAX_ROWS = 2097152
 
 # Every symbol include/dim_gpu.h declares (checked by the CPU test suite).
 EXPORTS = [
     "dim_last_error", "dim_status_string", "dim_version",
     "dim_csr5_from_csr", "dim_csr5_from_csr_device", "dim_csr5_destroy", "dim_csr5_get_info",
     "dim_csr5_copy_format", "dim_csr5_spmv", "dim_csr5_spmv_timed", "dim_csr5_block_rows", "dim_csr5_spmv_block", "dim_csr5_spmm",
-    "dim_csr5_spmm_colmajor", "dim_csr5_spmv_transpose", "dim_csr5_sampled_outer", "dim_cg", "dim_cg_multi", "dim_csr5_from_csr_batched",
+    "dim_csr5_spmm_colmajor", "dim_csr5_spmv_transpose", "dim_csr5_sampled_outer", "dim_cg", "dim_cg_noalloc",
+    "dim_cg_multi", "dim_csr5_from_csr_batched",
     "dim_cg_batched", "dim_csr5_n_systems", "dim_csr5_set_values",
 ]
 
@@ -99,6 +101,7 @@ def lib():
         "dim_csr5_spmv_transpose": ([vp, i32, vp, vp, vp], i32),
         "dim_csr5_sampled_outer": ([vp, f64, vp, vp, vp, vp], i32),
         "dim_cg": ([vp, vp, vp, P(CgOpts), P(CgResult), vp], i32),
+        "dim_cg_noalloc": ([vp, vp, vp, P(CgOpts), P(CgResult), vp], i32),
         "dim_cg_multi": ([vp, u32, vp, vp, P(CgOpts), P(CgResult), vp], i32),
         "dim_csr5_from_csr_batched": ([u32, vp, vp, vp, vp, vp, P(Csr5Opts), P(vp)], i32),
         "dim_cg_batched": ([vp, vp, vp, P(CgOpts), P(CgResult), vp], i32),

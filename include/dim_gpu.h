@@ -194,6 +194,24 @@ typedef struct dim_cg_result {
 dim_status dim_cg(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts, dim_cg_result* res,
                   void* stream);
 
+/* The solve of dim_cg (one right-hand side, DIM_CG_F64 or DIM_CG_F32_REFINED)
+ * for callers that cannot allocate device memory: loops that re-solve many
+ * times, harnesses that forbid hipMalloc. No counterpart in the reference.
+ * It calls no hipMalloc, hipFree or other allocating runtime routine (no
+ * pinned memory, graphs or events): its state, reduction partials and work
+ * vectors live in one static device array of DIM_CG_NOALLOC_BYTES (48.1 MiB)
+ * that every GPU the library runs on holds until the process ends, and the
+ * product's own carries are the handle's (allocated with the matrix).
+ * Limits: rows <= DIM_CG_NOALLOC_MAX_ROWS (else DIM_INVALID_ARGUMENT),
+ * opts->history NULL, the matrix on the current device. Calls that overlap
+ * on one GPU (other threads or streams) race on the array: serialise them.
+ * res->seconds is the host wall time of the call. Results equal dim_cg's
+ * bit for bit (same kernels in the same order). */
+#define DIM_CG_NOALLOC_MAX_ROWS 2097152u
+#define DIM_CG_NOALLOC_BYTES (1024u + 4u * 256u * 16u * 8u + 24u * DIM_CG_NOALLOC_MAX_ROWS + 8u * 256u)
+dim_status dim_cg_noalloc(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts,
+                          dim_cg_result* res, void* stream);
+
 /* k independent right-hand sides (B, X: rows x k row-major, DEVICE, FP64),
  * iterated together with one SpMM per iteration; res has k entries. */
 dim_status dim_cg_multi(const dim_csr5* A, uint32_t k, const double* B, double* X, const dim_cg_opts* opts,
