@@ -6,20 +6,32 @@ a 128^3 grid (n = 2,097,152, nnz = 14,581,760), x ~ U[-1,1] from a seed, with
 every input already resident in HBM. The headline value is the algorithmic
 bandwidth of that product, (8+4)*nnz + 8*(n_rows + n_cols) bytes per step,
 with the 256 MB Infinity Cache defeated by rotating over three copies of the
-matrix and vectors (625 MB); --warm adds the rate on one copy. The same JSON
-line carries the tile kernel's roofline figures, the reference CSR5
-AVX2/OpenMP path timed on this host's cores, FP64 CG on the same operator, the
-power-law SpMV, FP64 against FP32-refined multi-RHS CG on the 192^3
-variable-coefficient operator, and the batched CG with its adjoint gradient.
+matrix and vectors (625 MB). The same JSON line carries:
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-extras] [--warm]
+  * roofline: the tile kernel's algorithmic rate against the 8 TB/s peak, and
+    its HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE;
+  * the same product with the cache warm (one copy), with 32-bit column
+    indices (no column codes), and the measured stream-read ceiling;
+  * cpu_baseline: the reference's own CSR5 AVX2/OpenMP SpMV and a textbook CG
+    over it, timed in a child process on this host's cores with OpenMP
+    threads bound (OMP_PROC_BIND=spread, OMP_PLACES=cores);
+  * extras: FP64 CG on the same operator, one rank's share (part 3 of 8) of
+    the 128^3 and power-law matrices, the power-law SpMV in FP64 and FP32,
+    FP64 against FP32-refined multi-RHS CG on the 192^3 variable-coefficient
+    operator, and the batched CG with its adjoint gradient; every FP32 result
+    with its error against the FP64 oracle.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-extras] [--extras cg,parts,...]
+    python bench.py --traffic-sweep   # the products rocprofv3 counter runs profile
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -28,6 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "SpMV GB/s (% HBM peak); CG secs to rel-res 1e-10, FP32-refined vs FP64"
 
 
 def log(msg):
@@ -37,6 +50,8 @@ def log(msg):
 def spmv_bytes(nnz, rows, cols, vb=8, ib=4, k=1):
     return (vb + ib) * nnz + k * vb * (rows + cols)
 
+
+# ----------------------------------------------------------------- GPU legs --
 
 def time_spmv(mats, xs, ys, steps, warmup):
     """ms per step over `steps` products rotating through the copies."""
@@ -70,57 +85,44 @@ def kernel_times(mats, xs, ys, reps):
     return float(np.mean(tiles)), float(np.mean(fix))
 
 
-def cpu_baseline(rows, cols, rp, ci, v, x):
-    """The reference's CSR5 AVX2/OpenMP SpMV (oracle/_ref, built from the
-    reference's own header) on this host, 5 warm-ups + 20 timed calls; falls
-    back to the oracle's C restatement when the reference build is absent."""
-    from oracle import oracle as O
+def stream_read_ceiling(gib=4, reps=5):
+    """GB/s of dim_stream_read over a `gib` GiB buffer (the best of `reps`)."""
+    import ctypes as C
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    if O.ref_available():
-        M = O.RefCsr5(rows, cols, rp, ci, v)
-        kind = "reference"
-    else:
-        M = O.Csr5(rows, cols, rp, ci, v)
-        kind = "port"
-    y = np.empty(rows)
-    for _ in range(5):
-        y = M.spmv(x, nthreads=threads)
-    ts = []
-    for _ in range(20):
-        t0 = time.perf_counter()
-        y = M.spmv(x, nthreads=threads)
-        ts.append(time.perf_counter() - t0)
-    med = float(np.median(ts))
-    return dict(value=round(spmv_bytes(len(v), rows, cols) / med / 1e9, 2), unit="GB/s", cores=threads, kind=kind,
-                sample=f"full 128^3 Poisson FP64 SpMV, 5 warm-up + 20 timed calls, median {med * 1e3:.3f} ms "
-                       f"(y zero-filled and fresh calibrator per call, not timed)"), y
-
-
-def bench_powerlaw(steps):
     import torch
 
-    from diploma_research_amd import Csr5, matgen
+    from diploma_research_amd import native as N
 
-    n = 4_000_000
-    rp, ci, v = matgen.powerlaw(n, seed=2024)
-    nnz = len(v)
-    out = {"n": n, "nnz": nnz}
-    A = Csr5.from_csr((n, n, rp, ci, v), sigma=0, keep_f32=True)
-    out["sigma"] = A.sigma
-    # the autotuner's choice of the column-sliced form (8 slices, one per XCD)
-    out["xcd_slices"] = A.info.xcd_slices
-    out["xcd_sigma"] = A.info.xcd_sigma
-    del ci
-    g = torch.Generator().manual_seed(7)
-    for dt, vb in ((torch.float64, 8), (torch.float32, 4)):
-        x = (torch.rand(n, generator=g, dtype=torch.float64) * 2 - 1).to(dt).cuda()
-        y = torch.empty(n, dtype=dt, device="cuda")
-        ms = time_spmv([A], [x], [y], steps, 3)
-        gbs = spmv_bytes(nnz, n, n, vb=vb) / ms / 1e6
-        key = "fp64" if vb == 8 else "fp32"
-        out[key] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 3)}
-    return out
+    n = gib * (1 << 30) // 8
+    buf = torch.ones(n, dtype=torch.float64, device="cuda")
+    out = torch.zeros(1, dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream()
+    best = 0.0
+    for _ in range(reps + 1):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        N.check(N.lib().dim_stream_read(C.c_void_p(buf.data_ptr()), n * 8, C.c_void_p(out.data_ptr()),
+                                        C.c_void_p(s.cuda_stream)))
+        e1.record(s)
+        torch.cuda.synchronize()
+        best = max(best, n * 8 / e0.elapsed_time(e1) / 1e6)
+    assert float(out) == float(n)  # every word was read
+    del buf
+    return round(best, 1)
+
+
+def traffic_for(sigma, codes):
+    """HBM bytes per launch of the headline kernel at `sigma` from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/)."""
+    tfile = os.path.join(ROOT, "profiles", "spmv_tiles_f64_poisson128_traffic.json")
+    if not os.path.exists(tfile):
+        return None
+    with open(tfile) as f:
+        t = json.load(f)
+    key = f"{sigma}{'c' if codes else ''}"
+    entry = t.get("by_kernel", {}).get(key) or t.get("by_sigma", {}).get(str(sigma))
+    return entry.get("hbm_bytes_per_launch") if entry else None
 
 
 def bench_cg(A, rows, rp, ci, v):
@@ -136,16 +138,88 @@ def bench_cg(A, rows, rp, ci, v):
     conjugate_gradient(A, b, rtol=1e-8, max_iters=5000)  # warm-up
     x, res = conjugate_gradient(A, b, rtol=1e-8, max_iters=5000)
     err = float(torch.linalg.vector_norm(x - xd) / torch.linalg.vector_norm(xd))
-    return {"workload": "3-D 7-pt Poisson 128^3, FP64, rtol 1e-8, max 5000", "seconds": round(res.seconds, 5),
+    nnz = len(v)
+    it_bytes = 12 * nnz + 11 * 8 * rows  # per iteration: matrix once + 11 vector passes
+    sec = res.seconds
+    return {"workload": "3-D 7-pt Poisson 128^3, FP64, b = A x*, rtol 1e-8, max 5000", "seconds": round(sec, 5),
             "iterations": res.iterations, "relres_true": res.relres_true, "rel_err_vs_xstar": err,
-            "ms_per_iteration": round(1e3 * res.seconds / max(res.iterations, 1), 4)}
+            "ms_per_iteration": round(1e3 * sec / max(res.iterations, 1), 4),
+            "GB/s_per_iteration": round(it_bytes * res.iterations / max(sec, 1e-12) / 1e9, 1)}
 
 
-def bench_mixed(side=192, k=8, rtol=1e-10):
+def bench_parts(copies_c2=8, copies_c3=3, steps=40, part=3, count=8):
+    """One rank's share: part `part` of `count` of the reference's tile
+    partition (ceil(T/count) tiles of 64 elements, libs/dim/io/h5.cpp:95-103)
+    of the 128^3 and power-law matrices, converted as a rank converts its
+    loaded part (autotuned) and multiplied with the cache defeated by
+    rotating copies. Bytes: 12 per element + 8 per row of the share + 8 per
+    distinct column its elements read (x read once)."""
+    import torch
+
+    from diploma_research_amd import Csr5, h5, matgen
+
+    out = {}
+    for name, gen, n, copies in (("poisson128", lambda: matgen.poisson3d(128), 128 ** 3, copies_c2),
+                                 ("powerlaw4m", lambda: matgen.powerlaw(4_000_000, seed=2024), 4_000_000, copies_c3)):
+        rp, ci, v = gen()
+        first, v0, (nr, nc, brp, bci, bv) = h5.partition_block(n, n, rp, ci, v, part, count)
+        del rp, ci, v
+        ucols = int(np.unique(bci).size)
+        A0 = Csr5.from_csr((nr, nc, brp, bci, bv), sigma=0)
+        sliced = bool(A0.info.xcd_slices)
+        mats = [A0] + [Csr5.from_csr((nr, nc, brp, bci, bv), sigma=A0.sigma, xcd_slices=sliced)
+                       for _ in range(copies - 1)]
+        x = torch.from_numpy(np.random.default_rng(3).uniform(-1, 1, n)).cuda()
+        xs = [x.clone() for _ in mats]
+        ys = [torch.empty(nr, dtype=torch.float64, device="cuda") for _ in mats]
+        ms = time_spmv(mats, xs, ys, steps, 5)
+        byts = 12 * len(bv) + 8 * (nr + ucols)
+        out[name] = {"part": f"{part} of {count}", "first_row": first, "rows": nr, "nnz": len(bv),
+                     "distinct_cols": ucols, "sigma": A0.sigma, "xcd_slices": A0.info.xcd_slices,
+                     "ms": round(ms, 5), "GB/s": round(byts / ms / 1e6, 1),
+                     "frac_hbm_peak": round(byts / ms / 1e6 / HBM_PEAK_GBS, 4), "copies_rotated": copies}
+        del mats, xs, ys, A0
+        torch.cuda.synchronize()
+    return out
+
+
+def bench_powerlaw(steps, keep):
+    """Power-law SpMV (n = 4M), FP64 and FP32, autotuned format; keeps both
+    results in `keep` for the comparison with the FP64 oracle."""
+    import torch
+
+    from diploma_research_amd import Csr5, matgen
+
+    n = 4_000_000
+    rp, ci, v = matgen.powerlaw(n, seed=2024)
+    nnz = len(v)
+    out = {"n": n, "nnz": nnz}
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=0, keep_f32=True)
+    out["sigma"] = A.sigma
+    # the autotuner's choice of the column-sliced form (8 slices, one per XCD)
+    out["xcd_slices"] = A.info.xcd_slices
+    out["xcd_sigma"] = A.info.xcd_sigma
+    del rp, ci, v
+    xh = np.random.default_rng(7).uniform(-1, 1, n)
+    for dt, vb in ((torch.float64, 8), (torch.float32, 4)):
+        x = torch.from_numpy(xh).to(dt).cuda()
+        y = torch.empty(n, dtype=dt, device="cuda")
+        ms = time_spmv([A], [x], [y], steps, 3)
+        gbs = spmv_bytes(nnz, n, n, vb=vb) / ms / 1e6
+        key = "fp64" if vb == 8 else "fp32"
+        t_tiles, t_rest = kernel_times([A], [x], [y], 3)
+        out[key] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 3),
+                    "tile_kernel_ms": round(t_tiles, 4), "rest_ms": round(t_rest, 4)}
+        keep[f"powerlaw_y_{key}"] = y.double().cpu().numpy()
+    return out
+
+
+def bench_mixed(keep, side=192, k=8, rtol=1e-10):
     """FP64 CG against FP32 CG inside FP64 residual replacement on the 3-D
     variable-coefficient operator (lognormal(0,1) diffusivity, harmonic-mean
     faces), k right-hand sides ~ U[-1,1] solved together (one SpMM per
-    iteration), x0 = 0, stop at relative residual rtol (FP64 true residual)."""
+    iteration), x0 = 0, stop at relative residual rtol (FP64 true residual).
+    Keeps column 0 of both solutions for the comparison with the oracle."""
     import torch
 
     from diploma_research_amd import Csr5, conjugate_gradient_multi, matgen
@@ -157,19 +231,23 @@ def bench_mixed(side=192, k=8, rtol=1e-10):
     del rp, ci, v
     B = torch.from_numpy(np.random.default_rng(5).uniform(-1, 1, (n, k))).cuda()
     out = {"workload": f"3-D variable-coefficient operator {side}^3 (n={n}, nnz={nnz}), lognormal(0,1), "
-                       f"k={k} RHS, rtol {rtol:g}", "sigma": A.sigma}
+                       f"k={k} RHS, rtol {rtol:g}", "sigma": A.sigma, "column_codes": bool(A.info.column_codes)}
     X = {}
     for prec in ("f64", "f32-refined"):
         Xp, res = conjugate_gradient_multi(A, B, rtol=rtol, max_iters=20000, precision=prec)
         torch.cuda.synchronize()
         X[prec] = Xp
-        out[prec] = {"seconds": round(max(r.seconds for r in res), 4),
-                     "iterations": [r.iterations for r in res],
+        sec = max(r.seconds for r in res)
+        its = max(r.iterations for r in res)
+        vb = 8 if prec == "f64" else 4
+        it_bytes = (vb + 4) * nnz + k * 11 * vb * n
+        out[prec] = {"seconds": round(sec, 4), "iterations": [r.iterations for r in res],
                      "relres_true_max": max(r.relres_true for r in res),
-                     "converged": all(r.converged for r in res),
-                     "refinements": res[0].refinements}
+                     "converged": all(r.converged for r in res), "refinements": res[0].refinements,
+                     "GB/s_per_iteration": round(it_bytes * its / sec / 1e9, 1)}
+        keep[f"mixed_x0_{prec}"] = Xp[:, 0].cpu().numpy()
     err = torch.linalg.vector_norm(X["f32-refined"] - X["f64"], dim=0) / torch.linalg.vector_norm(X["f64"], dim=0)
-    out["f32-refined"]["max_rel_err_vs_fp64_solution"] = float(err.max())
+    out["f32-refined"]["max_rel_err_vs_fp64_gpu_solution"] = float(err.max())
     out["speedup_f32_refined_vs_f64"] = round(out["f64"]["seconds"] / out["f32-refined"]["seconds"], 3)
     return out
 
@@ -179,12 +257,9 @@ def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-6, h4=1e-3, rt
     [lo, hi], lognormal(0, 0.5)) solved in one batched FP64 CG launch
     sequence; L = 0.5 ||x||^2; dL/db and dL/dvals from the adjoint (batched CG
     with A's own product, A being symmetric, + sampled outer product) through
-    torch.autograd; central differences
-    on n_fd random stored entries, each a set of re-solves of its system:
-    2nd order with step h, and 4th order with step h4. With h = 1e-6 the
-    solves' rounding (rtol 1e-15) shows in the quotient at about 1e-6 for
-    some entries; with h4 = 1e-3 it stays far below, and the 4th-order
-    stencil's truncation error (of order h4^4) too."""
+    torch.autograd; central differences on n_fd random stored entries, each a
+    set of re-solves of its system: 2nd order with step h, and 4th order with
+    step h4."""
     import torch
 
     from diploma_research_amd import Csr5, conjugate_gradient, matgen
@@ -263,20 +338,174 @@ def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-6, h4=1e-3, rt
                     "within_1e-6": int((errs4 <= 1e-6).sum())}}
 
 
+# ------------------------------------------------------- CPU baseline leg --
+
+def host_cores():
+    """(threads to use, physical cores lscpu reports, CPUs this process may
+    run on). The thread count is OMP_NUM_THREADS when the host sets it (the
+    GPU box sets its share), else the physical cores, within the affinity."""
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], stdout=subprocess.PIPE, text=True, timeout=10).stdout
+        pairs = {tuple(l.split(",")) for l in out.splitlines() if l and not l.startswith("#")}
+        phys = len(pairs) or None
+    except Exception:
+        pass
+    avail = len(os.sched_getaffinity(0))
+    want = int(os.environ.get("OMP_NUM_THREADS", "0")) or (phys or avail)
+    return max(1, min(want, avail)), phys, avail
+
+
+def cpu_leg(outdir, with_c4=True):
+    """The CPU baseline leg, run in a child process whose OpenMP threads are
+    bound (OMP_PROC_BIND=spread, OMP_PLACES=cores, set before the runtime
+    starts). Times the reference's CSR5 SpMV (oracle/_ref, built from its own
+    header; the oracle's C restatement when that build is absent) on the
+    128^3 operator with the reference's benchmark protocol (5 warm-up + 20
+    timed products), and a textbook CG over it to rtol 1e-8. As the checker
+    of the GPU results it also writes the FP64 oracle's power-law product
+    and, with with_c4, the oracle's CG solution of column 0 of the 192^3
+    variable-coefficient system to `outdir`."""
+    from diploma_research_amd import matgen
+    from oracle import oracle as O
+
+    threads = int(os.environ["OMP_NUM_THREADS"])
+    log(f"CPU leg: {threads} threads")
+    n = 128 ** 3
+    rp, ci, v = matgen.poisson3d(128)
+    x = np.random.default_rng(1).uniform(-1, 1, n)
+    kind = "reference" if O.ref_available() else "port"
+    res = {"threads": threads, "kind": kind}
+    if kind == "reference":
+        M = O.RefCsr5(n, n, rp, ci, v)
+        ts, y = M.time_spmv(x, threads, warmup=5, runs=20)
+    else:
+        M = O.Csr5(n, n, rp, ci, v)
+        ts = []
+        for i in range(25):
+            t0 = time.perf_counter()
+            y = M.spmv(x, nthreads=threads)
+            if i >= 5:
+                ts.append(time.perf_counter() - t0)
+        ts = np.array(ts)
+    np.save(os.path.join(outdir, "c2_y.npy"), y)
+    med = float(np.median(ts))
+    res["spmv"] = {"median_ms": round(med * 1e3, 4), "mean_ms": round(float(np.mean(ts)) * 1e3, 4),
+                   "GB/s": round(spmv_bytes(len(v), n, n) / med / 1e9, 2)}
+    xs = np.random.default_rng(42).uniform(-1, 1, n)
+    b = O.csr_spmv(rp, ci, v, xs)
+    if kind == "reference":
+        cg = M.cg(b, rtol=1e-8, maxit=5000, nthreads=threads)
+    else:
+        t0 = time.perf_counter()
+        cg = M.cg(b, rtol=1e-8, maxit=5000, nthreads=threads)
+        cg["seconds"] = time.perf_counter() - t0
+    res["cg"] = {"seconds": round(cg["seconds"], 4), "iterations": cg["iterations"],
+                 "relres_true": cg["relres_true"], "converged": cg["converged"]}
+    del M
+    log("CPU leg: SpMV and CG timed; oracle products for the checks")
+    # checker outputs (not timed)
+    m = 4_000_000
+    rp, ci, v = matgen.powerlaw(m, seed=2024)
+    xp = np.random.default_rng(7).uniform(-1, 1, m)
+    np.save(os.path.join(outdir, "powerlaw_y.npy"), O.csr_spmv(rp, ci, v, xp))
+    lens = np.diff(rp.astype(np.int64))
+    np.save(os.path.join(outdir, "powerlaw_absum.npy"),
+            np.bincount(np.repeat(np.arange(m), lens), weights=np.abs(v * xp[ci]), minlength=m))
+    np.save(os.path.join(outdir, "powerlaw_lens.npy"), lens)
+    del rp, ci, v
+    if with_c4:
+        side = 192
+        nc = side ** 3
+        rp, ci, v = matgen.varcoef((side, side, side), 1.0, seed=4)
+        b0 = np.random.default_rng(5).uniform(-1, 1, (nc, 8))[:, 0].copy()
+        log("CPU leg: oracle CG of column 0 of the 192^3 system (about a minute)")
+        t0 = time.perf_counter()
+        Mc = O.RefCsr5(nc, nc, rp, ci, v) if kind == "reference" else O.Csr5(nc, nc, rp, ci, v)
+        oc = Mc.cg(b0, rtol=1e-10, maxit=20000, nthreads=threads)
+        np.save(os.path.join(outdir, "c4_x0.npy"), oc["x"])
+        res["c4_oracle_cg"] = {"iterations": oc["iterations"], "relres_true": oc["relres_true"],
+                               "seconds": round(time.perf_counter() - t0, 2)}
+    print(json.dumps(res), flush=True)
+
+
+def run_cpu_leg(with_c4):
+    """Starts the CPU leg in a child process with bound OpenMP threads;
+    returns (its JSON, the folder holding its checker outputs)."""
+    threads, phys, avail = host_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+    outdir = tempfile.mkdtemp(prefix="dim_cpu_leg_")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", outdir] + ([] if with_c4 else ["--no-oracle-c4"])
+    p = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, timeout=1500)
+    if p.returncode != 0:
+        raise RuntimeError(f"CPU leg failed with status {p.returncode}")
+    res = json.loads(p.stdout.strip().splitlines()[-1])
+    res["binding"] = {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "spread", "OMP_PLACES": "cores",
+                      "lscpu_physical_cores": phys, "cpus_in_affinity": avail}
+    return res, outdir
+
+
+# -------------------------------------------------------- traffic sweep --
+
+def traffic_sweep(copies=3, reps=6):
+    """The products that the rocprofv3 counter runs profile: the 128^3
+    Poisson SpMV at every tile height (column-coded and 32-bit-column kernels
+    where both exist), `reps` launches each rotating over `copies` copies,
+    then the power-law product in its autotuned form, and a 256 MiB device
+    copy of known size that scales the counters."""
+    import torch
+
+    from diploma_research_amd import Csr5, matgen
+
+    n = 128 ** 3
+    rp, ci, v = matgen.poisson3d(128)
+    x = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, n)).cuda()
+    for sigma in (4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32):
+        for plain in ((False, True) if sigma <= 8 else (False,)):
+            mats = [Csr5.from_csr((n, n, rp, ci, v), sigma=sigma, plain_columns=plain) for _ in range(copies)]
+            xs = [x.clone() for _ in mats]
+            ys = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in mats]
+            time_spmv(mats, xs, ys, reps, 0)
+            log(f"sigma {sigma} {'plain' if plain else 'coded' if mats[0].info.column_codes else ''} done")
+            del mats, xs, ys
+    del rp, ci, v
+    m = 4_000_000
+    rp, ci, v = matgen.powerlaw(m, seed=2024)
+    A = Csr5.from_csr((m, m, rp, ci, v), sigma=0, keep_f32=True)
+    log(f"power-law: sigma {A.sigma}, slices {A.info.xcd_slices} (stacked sigma {A.info.xcd_sigma})")
+    xp = torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, m)).cuda()
+    yp = torch.empty(m, dtype=torch.float64, device="cuda")
+    for _ in range(reps):
+        A.spmv(xp, yp)
+    torch.cuda.synchronize()
+    src = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("traffic sweep done")
+
+
+# ------------------------------------------------------------------ main --
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-extras", action="store_true", help="SpMV headline only")
-    ap.add_argument("--extras", default="cg,powerlaw,mixed,batched",
-                    help="comma-separated extras to run after the headline (cg, powerlaw, mixed, batched)")
-    ap.add_argument("--warm", action="store_true",
-                    help="also time the product on one matrix copy over and over (Infinity Cache warm)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain 256 MiB device-to-device copy (3 times), the known-size reference for "
-                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--no-extras", action="store_true", help="SpMV headline and CPU baseline only")
+    ap.add_argument("--extras", default="cg,parts,powerlaw,mixed,batched",
+                    help="comma-separated extras after the headline (cg, parts, powerlaw, mixed, batched)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-oracle-c4", action="store_true",
+                    help="skip the oracle's CG solve of the 192^3 system (about a minute of CPU time)")
+    ap.add_argument("--traffic-sweep", action="store_true",
+                    help="run the products the rocprofv3 counter runs profile, and nothing else")
+    ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.cpu_leg:
+        cpu_leg(a.cpu_leg, with_c4=not a.no_oracle_c4)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the CSR5 path runs on a single MI355X here "
               "(multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
@@ -288,6 +517,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.traffic_sweep:
+        traffic_sweep()
+        return
     side = 128
     n = side ** 3
     log("generating the 128^3 7-point Poisson operator")
@@ -314,38 +546,33 @@ def main():
     single_kernel = t_fix == 0.0
     kernel_ms = ms if single_kernel else t_tiles
     achieved = bytes_step / kernel_ms / 1e6
-    gbs_warm = None
-    if a.warm:
-        gbs_warm = round(bytes_step / time_spmv(mats[:1], xs[:1], ys[:1], a.steps, a.warmup) / 1e6, 2)
-
-
-    # HBM bytes per launch of this kernel from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "spmv_tiles_f64_poisson128_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("by_sigma", {}).get(str(sigma), {}).get("hbm_bytes_per_launch")
-
-    if a.copy_calibration:
-        src = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-
-    log("CPU baseline (reference CSR5 on host cores)")
-    cpu, y_cpu = cpu_baseline(n, n, rp, ci, v, x_host)
-    # parity of the timed product against the CPU baseline's own result
-    y = ys[0].cpu().numpy()
-    parity = float(np.linalg.norm(y - y_cpu) / np.linalg.norm(y_cpu))
+    y_gpu = ys[(a.steps - 1) % len(mats)].cpu().numpy()
+    gbs_warm = round(bytes_step / time_spmv(mats[:1], xs[:1], ys[:1], a.steps, a.warmup) / 1e6, 2)
+    del mats, xs, ys
+    # the same product with 32-bit column indices (no column codes), the form
+    # every matrix that is not a stencil uses
+    P0 = Csr5.from_csr((n, n, rp, ci, v), sigma=0, plain_columns=True)
+    pmats = [P0] + [Csr5.from_csr((n, n, rp, ci, v), sigma=P0.sigma, plain_columns=True) for _ in range(2)]
+    pxs = [torch.from_numpy(x_host).to(dev) for _ in pmats]
+    pys = [torch.empty(n, dtype=torch.float64, device=dev) for _ in pmats]
+    ms_plain = time_spmv(pmats, pxs, pys, a.steps, a.warmup)
+    plain = {"sigma": P0.sigma, "ms": round(ms_plain, 5), "GB/s": round(bytes_step / ms_plain / 1e6, 2),
+             "frac_hbm_peak": round(bytes_step / ms_plain / 1e6 / HBM_PEAK_GBS, 4),
+             "traffic": traffic_for(P0.sigma, False)}
+    del pmats, pxs, pys, P0
+    ceiling = stream_read_ceiling()
+    traffic = traffic_for(sigma, codes)
 
     extras = {}
+    keep = {}
     want = set() if a.no_extras else {w.strip() for w in a.extras.split(",") if w.strip()}
-    runs = [("cg", "cg_fp64", "FP64 CG on the 128^3 operator", lambda: bench_cg(mats[0], n, rp, ci, v)),
-            ("powerlaw", "powerlaw_spmv", "power-law SpMV (n = 4M)", lambda: bench_powerlaw(max(10, a.steps // 2))),
-            ("mixed", "cg_mixed_vs_fp64", "FP64 vs FP32-refined CG, 192^3 variable coefficients, k = 8", bench_mixed),
+    runs = [("cg", "cg_fp64", "FP64 CG on the 128^3 operator", lambda: bench_cg(A0, n, rp, ci, v)),
+            ("parts", "rank_share", "one rank's share (part 3 of 8) of the 128^3 and power-law matrices",
+             bench_parts),
+            ("powerlaw", "powerlaw_spmv", "power-law SpMV (n = 4M)",
+             lambda: bench_powerlaw(max(10, a.steps // 2), keep)),
+            ("mixed", "cg_mixed_vs_fp64", "FP64 vs FP32-refined CG, 192^3 variable coefficients, k = 8",
+             lambda: bench_mixed(keep)),
             ("batched", "batched_adjoint", "batched CG + adjoint, 32 uneven systems", bench_batched_adjoint)]
     for key, name, what, fn in runs:
         if key not in want:
@@ -355,9 +582,56 @@ def main():
             extras[name] = fn()
         except Exception as e:  # report, never hide
             extras[name] = {"error": f"{type(e).__name__}: {e}"}
+    del A0
+    torch.cuda.synchronize()
+
+    cpu = None
+    parity = None
+    if not a.no_cpu:
+        log("CPU leg: the reference CSR5 SpMV and CG on host cores, then the oracle's checker outputs")
+        with_c4 = "mixed" in want and not a.no_oracle_c4
+        try:
+            cres, outdir = run_cpu_leg(with_c4)
+            y_cpu = np.load(os.path.join(outdir, "c2_y.npy"))
+            parity = float(np.linalg.norm(y_gpu - y_cpu) / np.linalg.norm(y_cpu))
+            sp = cres["spmv"]
+            cpu = {"value": sp["GB/s"], "unit": "GB/s", "cores": cres["threads"], "kind": cres["kind"],
+                   "sample": f"full 128^3 Poisson FP64 SpMV, 5 warm-up + 20 timed products (only csr5::spmv "
+                             f"timed; y zero-filled and a fresh calibrator before each), median "
+                             f"{sp['median_ms']} ms, mean {sp['mean_ms']} ms",
+                   "binding": cres["binding"],
+                   "cg": dict(cres["cg"], workload="textbook CG over the reference's SpMV on the 128^3 operator, "
+                                                   "b = A x*, rtol 1e-8, max 5000")}
+            if "cg_fp64" in extras and "iterations" in extras["cg_fp64"]:
+                g = extras["cg_fp64"]
+                g["cpu_seconds"] = cres["cg"]["seconds"]
+                g["cpu_iterations"] = cres["cg"]["iterations"]
+                g["speedup_vs_cpu"] = round(cres["cg"]["seconds"] / g["seconds"], 1)
+            pl = extras.get("powerlaw_spmv", {})
+            if "powerlaw_y_fp64" in keep:
+                ref = np.load(os.path.join(outdir, "powerlaw_y.npy"))
+                absum = np.load(os.path.join(outdir, "powerlaw_absum.npy"))
+                lens = np.load(os.path.join(outdir, "powerlaw_lens.npy"))
+                u32 = np.finfo(np.float32).eps / 2
+                for key in ("fp64", "fp32"):
+                    yk = keep[f"powerlaw_y_{key}"]
+                    entry = {"normwise_rel_err_vs_fp64_oracle": float(np.linalg.norm(yk - ref) / np.linalg.norm(ref))}
+                    if key == "fp32":
+                        entry["rows_within_fp32_bound"] = bool(
+                            (np.abs(yk - ref) <= (lens + 4) * u32 * absum + 1e-30).all())
+                    pl[key].update(entry)
+            mx = extras.get("cg_mixed_vs_fp64", {})
+            if "c4_oracle_cg" in cres and "mixed_x0_f64" in keep:
+                xo = np.load(os.path.join(outdir, "c4_x0.npy"))
+                mx["oracle_cg_column0"] = cres["c4_oracle_cg"]
+                for prec in ("f64", "f32-refined"):
+                    mx[prec]["column0_rel_err_vs_fp64_oracle_x"] = float(
+                        np.linalg.norm(keep[f"mixed_x0_{prec}"] - xo) / np.linalg.norm(xo))
+        except Exception as e:  # report, never hide
+            cpu = {"error": f"{type(e).__name__}: {e}"}
 
     line = {
-        "metric": "SpMV GB/s (% HBM peak); CG secs to rel-res 1e-10, FP32-refined vs FP64",
+        "metric": METRIC,
         "value": round(gbs, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -373,6 +647,8 @@ def main():
                                "x~U[-1,1]; Infinity Cache defeated by rotating 3 matrix copies",
                    "sigma": sigma, "omega": 64, "bytes_per_step": bytes_step, "column_codes": codes,
                    "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "warm_GB/s": gbs_warm,
+                   "plain_columns": plain, "stream_read_ceiling_GB/s": ceiling,
+                   "frac_stream_ceiling": round(gbs / ceiling, 4) if ceiling else None,
                    "parity_rel_err_vs_cpu_baseline": parity},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

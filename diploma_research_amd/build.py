@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["format.cpp", "capi.cpp", "spmv.hip", "spmv_k2.hip", "spmv_k4.hip", "spmv_k8.hip", "spmv_k16.hip",
-           "cg.hip", "batched.hip", "convert.hip"]
+           "cg.hip", "batched.hip", "convert.hip", "probe.hip"]
 
 # Packed FP32 instructions are switched off for all device code (some of their
 # forms have given wrong results on these machines).

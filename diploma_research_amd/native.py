@@ -37,7 +37,7 @@ EXPORTS = [
     "dim_csr5_copy_format", "dim_csr5_spmv", "dim_csr5_spmv_timed", "dim_csr5_block_rows", "dim_csr5_spmv_block", "dim_csr5_spmm",
     "dim_csr5_spmm_colmajor", "dim_csr5_spmv_transpose", "dim_csr5_sampled_outer", "dim_cg", "dim_cg_noalloc",
     "dim_cg_multi", "dim_csr5_from_csr_batched",
-    "dim_cg_batched", "dim_csr5_n_systems", "dim_csr5_set_values",
+    "dim_cg_batched", "dim_csr5_n_systems", "dim_csr5_set_values", "dim_stream_read",
 ]
 
 
@@ -107,6 +107,7 @@ def lib():
         "dim_cg_batched": ([vp, vp, vp, P(CgOpts), P(CgResult), vp], i32),
         "dim_csr5_n_systems": ([vp], u32),
         "dim_csr5_set_values": ([vp, vp, vp], i32),
+        "dim_stream_read": ([vp, u64, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

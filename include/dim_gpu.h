@@ -241,6 +241,13 @@ uint32_t dim_csr5_n_systems(const dim_csr5* A);
  * counterpart; used when differentiating with respect to A. */
 dim_status dim_csr5_set_values(dim_csr5* A, const double* vals, void* stream);
 
+/* Measurement helper with no reference counterpart: reads `bytes` (a multiple
+ * of 16, 16-byte aligned) of DEVICE memory at `data` once with non-temporal
+ * 16-byte loads and writes their sum to the DEVICE double *sum, on `stream`.
+ * Timing it gives the HBM read rate the GPU reaches on a plain stream, the
+ * measured ceiling reported beside the 8 TB/s peak. */
+dim_status dim_stream_read(const void* data, uint64_t bytes, double* sum, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,9 @@ def ref():
         L.dimref_csr5_tail_partition_start.restype = C.c_uint32
         L.dimref_csr5_copy.argtypes = [C.c_void_p, _u32p, _u32p, _u32p, _u32p, _f64p, _u32p]
         L.dimref_csr5_spmv.argtypes = [C.c_void_p, C.c_int, _f64p, C.c_uint64, _f64p, C.c_uint64]
+        L.dimref_csr5_time_spmv.argtypes = [C.c_void_p, C.c_int, _f64p, C.c_uint64, _f64p, C.c_uint64, C.c_int,
+                                            C.c_int, _f64p]
+        L.dimref_cg.argtypes = [C.c_void_p, C.c_int, _f64p, _f64p, C.c_double, C.c_uint32, _f64p]
         L.dimref_session_new.argtypes = [C.c_void_p, C.c_int]
         L.dimref_session_new.restype = C.c_void_p
         L.dimref_session_free.argtypes = [C.c_void_p]
@@ -234,6 +237,25 @@ class RefCsr5:
         if ref().dimref_csr5_spmv(self._h, nthreads, x, len(x), y, len(y)):
             raise ValueError("can't multiply, vector size doesn't match column count of the matrix")
         return y
+
+    def time_spmv(self, x, nthreads, warmup=5, runs=20):
+        """The reference's benchmark protocol: seconds of each of `runs`
+        products after `warmup` (only csr5::spmv is timed; y is zeroed and the
+        calibrator fresh before each). Returns (times, y)."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty(self.rows)
+        t = np.zeros(runs)
+        ref().dimref_csr5_time_spmv(self._h, int(nthreads), x, len(x), y, len(y), int(warmup), int(runs), t)
+        return t, y
+
+    def cg(self, b, x0=None, rtol=1e-8, maxit=5000, nthreads=8):
+        """Textbook CG with the reference's SpMV (dimref_cg): returns dict of
+        x, iterations, converged, relres, relres_true and seconds."""
+        x = np.zeros(self.rows) if x0 is None else np.array(x0, np.float64)
+        out = np.zeros(5)
+        ref().dimref_cg(self._h, int(nthreads), np.ascontiguousarray(b, np.float64), x, float(rtol), int(maxit), out)
+        return dict(x=x, iterations=int(out[0]), converged=bool(out[1]), relres=float(out[2]),
+                    relres_true=float(out[3]), seconds=float(out[4]))
 
     def session(self, nthreads):
         """The reference's own calling convention: one y and one calibrator reused."""
