@@ -264,6 +264,7 @@ def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-6, h4=1e-3, rt
 
     from diploma_research_amd import Csr5, conjugate_gradient, matgen
     from diploma_research_amd.autograd import solve
+    from diploma_research_amd.csr5 import conjugate_gradient_batched
 
     rng = np.random.default_rng(2024)
     sides = rng.integers(lo, hi + 1, size=n_sys)
@@ -286,6 +287,16 @@ def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-6, h4=1e-3, rt
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     gv = vals.grad.cpu().numpy()
+    # the forward solve alone in both forms: the captured launch sequence
+    # (default) and the persistent kernel with grid barriers
+    bd = b.detach()
+    forms = {}
+    for name, pers in (("launch_sequence", False), ("persistent", True)):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        xf, rf = conjugate_gradient_batched(A, bd, rtol=rtol, max_iters=50000, persistent=pers)
+        torch.cuda.synchronize()
+        forms[name] = (time.perf_counter() - ta, xf, max(r.iterations for r in rf))
 
     # central differences, each perturbing a_ij and a_ji together (symmetric
     # systems: the derivative of that parametrisation is g_ij + g_ji)
@@ -325,11 +336,13 @@ def bench_batched_adjoint(n_sys=32, lo=100, hi=700, n_fd=64, h=1e-6, h4=1e-3, rt
         errs.append(abs(fd2 - an) / max(abs(an), 1e-300))
         errs4.append(abs(fd4 - an) / max(abs(an), 1e-300))
     t4 = time.perf_counter()
-    errs, errs4 = np.array(errs), np.array(errs4)
+    errs, errs4 = np.array(errs + [0.0] * (not errs)), np.array(errs4 + [0.0] * (not errs4))
     return {"workload": f"{n_sys} 2-D variable-coefficient systems, sides {int(sides.min())}..{int(sides.max())} "
                         f"(n={int(sum(n for n, *_ in systems))}, nnz={len(vals_np)}), lognormal(0,0.5), FP64, "
                         f"rtol {rtol:g}",
             "sigma": A.sigma, "forward_seconds": round(t1 - t0, 4), "backward_seconds": round(t2 - t1, 4),
+            "cg_forms": {k: {"seconds": round(v[0], 4), "max_iterations": v[2]} for k, v in forms.items()},
+            "cg_forms_bitwise_equal": bool(torch.equal(forms["launch_sequence"][1], forms["persistent"][1])),
             "fd": {"entries": n_fd, "h": h, "stencil": "central, 2nd order", "seconds": round(t4 - t3, 2),
                    "rel_err_median": float(np.median(errs)), "rel_err_max": float(errs.max()),
                    "within_1e-6": int((errs <= 1e-6).sum())},
@@ -499,6 +512,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-oracle-c4", action="store_true",
                     help="skip the oracle's CG solve of the 192^3 system (about a minute of CPU time)")
+    ap.add_argument("--fd-entries", type=int, default=64,
+                    help="stored entries the batched adjoint is checked on by finite differences (0: none)")
     ap.add_argument("--traffic-sweep", action="store_true",
                     help="run the products the rocprofv3 counter runs profile, and nothing else")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
@@ -573,7 +588,8 @@ def main():
              lambda: bench_powerlaw(max(10, a.steps // 2), keep)),
             ("mixed", "cg_mixed_vs_fp64", "FP64 vs FP32-refined CG, 192^3 variable coefficients, k = 8",
              lambda: bench_mixed(keep)),
-            ("batched", "batched_adjoint", "batched CG + adjoint, 32 uneven systems", bench_batched_adjoint)]
+            ("batched", "batched_adjoint", "batched CG + adjoint, 32 uneven systems",
+             lambda: bench_batched_adjoint(n_fd=a.fd_entries))]
     for key, name, what, fn in runs:
         if key not in want:
             continue

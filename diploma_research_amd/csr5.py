@@ -464,18 +464,20 @@ def conjugate_gradient_multi(A: Csr5, B, X=None, rtol=1e-8, max_iters=5000, prec
 
 
 def conjugate_gradient_batched(A: Csr5, B, X=None, rtol=1e-8, max_iters=5000, check_every=0, transpose=False,
-                               stream=None):
+                               stream=None, persistent=False):
     """Solves every system of a batch (A from Csr5.from_csr_batched; B, X:
     float64 GPU tensors holding the systems' vectors back to back). With
-    transpose=True the products use A^T (the adjoint solve). Returns (X,
-    [CgResult per system])."""
+    transpose=True the products use A^T (the adjoint solve). persistent=True
+    runs the iteration loop in one cooperative kernel with grid barriers
+    (not with transpose). Returns (X, [CgResult per system])."""
     torch = _torch()
     n = A.dimensions.rows  # the systems' rows together
     _check_tensor(B, "B", n)
     if X is None:
         X = torch.zeros_like(B)
     _check_tensor(X, "X", n)
-    opts = _cg_opts(rtol, max_iters, "f64", check_every, 0.0, None, N.DIM_CG_TRANSPOSE if transpose else 0)
+    flags = (N.DIM_CG_TRANSPOSE if transpose else 0) | (N.DIM_CG_PERSISTENT if persistent else 0)
+    opts = _cg_opts(rtol, max_iters, "f64", check_every, 0.0, None, flags)
     S = A.n_systems
     res = (N.CgResult * S)()
     N.check(N.lib().dim_cg_batched(A._h, _ptr(B), _ptr(X), C.byref(opts), res, _stream_ptr(stream)),

@@ -150,6 +150,11 @@ DevFormat system_view(const dim_csr5* A, uint32_t s) {
     view.vals64 = A->f.vals64 + t0 * tile;
     view.vals32 = A->f.vals32 ? A->f.vals32 + t0 * tile : nullptr;
     view.col_code = A->f.col_code ? A->f.col_code + size_t{t0} * kOmega : nullptr;
+    // a batch with column codes keeps one table of diagonals per system
+    view.col_dict = A->f.col_dict && A->n_systems > 1 ? A->f.col_dict + size_t{s} * kMaxDiagonals : A->f.col_dict;
+    view.chunk_tile = nullptr;
+    view.chunk_sys = nullptr;
+    view.n_chunk_tab = 0;
     view.nnz = A->sys_nnz_csr[s + 1] - A->sys_nnz_csr[s];
     view.padded = size_t{view.tiles} * tile;
     return view;
@@ -651,6 +656,8 @@ dim_status dim_csr5_spmv_block(const dim_csr5* A, dim_dtype dtype, uint32_t tile
         return st;
     if (x_len != A->f.cols)
         return fail(DIM_INVALID_ARGUMENT, "can't multiply, vector size doesn't match column count of the matrix");
+    if (A->n_systems > 1 && A->f.col_code)
+        return fail(DIM_UNSUPPORTED, "row-block products of a batch with per-system column codes");
     uint32_t first = 0, n = 0;
     if (dim_status st = dim_csr5_block_rows(A, tile_begin, tile_end, &first, &n))
         return st;

@@ -23,6 +23,9 @@ struct dim_csr5 {
     uint32_t n_systems = 1;
     std::vector<uint32_t> sys_row, sys_tile;
     std::vector<uint64_t> sys_nnz_csr;
+    // chunks of system s in the chunk table: [sys_chunk[s], sys_chunk[s + 1])
+    std::vector<uint32_t> sys_chunk;
+    uint32_t* d_sys_chunk = nullptr;
     uint64_t nnz_logical = 0; // stored elements (the format's nnz counts padding inside a batch)
     dimgpu::BoundaryInfo binfo; // host copy of the tile flags (f.binfo points here)
     // Column slices for plain products (capi.cpp): xs is the matrix whose

@@ -22,6 +22,8 @@ class BoundaryInfo {
     }
     // tiles [tb, te) of the flag array, chunks of tpw tiles
     bool needs_fixup(uint32_t tb, uint32_t te, uint32_t tpw) const;
+    // the same for chunks at the boundaries chunk_tile[0..n]
+    bool needs_fixup_table(const std::vector<uint32_t>& chunk_tile) const;
 
   private:
     std::vector<uint8_t> flags_;
@@ -53,6 +55,15 @@ struct DevFormat {
     // binfo_base + t there
     const BoundaryInfo* binfo = nullptr;
     uint32_t binfo_base = 0;
+    // Block-diagonal batches: the chunk table (chunk c covers tiles
+    // [chunk_tile[c], chunk_tile[c + 1]) of system chunk_sys[c]; chunks never
+    // hold two systems), used by products over the whole matrix, and whether
+    // some chunk boundary needs the fix-up kernel. With column codes col_dict
+    // holds kMaxDiagonals entries per system.
+    const uint32_t* chunk_tile = nullptr;
+    const uint32_t* chunk_sys = nullptr;
+    uint32_t n_chunk_tab = 0;
+    bool chunk_tab_fixup = true;
 };
 
 // Per-matrix scratch: tile carries and reduction partials (the GPU
@@ -75,10 +86,9 @@ struct Scratch {
 // partial per fix-up block, or scratch.dot_chunk, one per chunk, when a
 // single-vector product needs no fix-up kernel) and *dot_blocks (if given)
 // the number of partials.
-// tile_sys, sys_done (single-vector products without a dot; may be null):
-// the system of each tile of a block-diagonal batch and the systems' stop
-// flags; chunks whose tiles all belong to stopped systems leave their rows
-// of y as they are.
+// sys_done (single-vector products of a batch over the whole matrix; may be
+// null): the systems' stop flags; the chunks of stopped systems leave their
+// rows of y as they are.
 // `done` (may be null): a device flag; when *done != 0 the kernels return at
 // once (used by solver launch sequences).
 // ldx: 0 for row-major n x k blocks (element (i, j) at i*k + j); otherwise the
@@ -88,8 +98,7 @@ hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const vo
                        uint32_t tile_begin, uint32_t tile_end, uint32_t row_base, uint32_t empty_begin,
                        uint32_t empty_end, bool dot, const int* done, hipStream_t stream,
                        uint32_t* dot_blocks = nullptr, uint64_t ldx = 0, uint64_t ldy = 0,
-                       const double** dot_parts = nullptr, const uint32_t* tile_sys = nullptr,
-                       const int* sys_done = nullptr);
+                       const double** dot_parts = nullptr, const int* sys_done = nullptr);
 
 // Same as launch_spmv over the whole matrix with events recorded before the
 // tile kernel, between the two kernels and after the fix-up kernel.

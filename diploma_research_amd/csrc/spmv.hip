@@ -177,6 +177,20 @@ bool BoundaryInfo::needs_fixup(uint32_t tb, uint32_t te, uint32_t tpw) const {
     return need;
 }
 
+bool BoundaryInfo::needs_fixup_table(const std::vector<uint32_t>& chunk_tile) const {
+    std::lock_guard<std::mutex> g(m_);
+    if (chunk_tile.empty() || chunk_tile.back() > flags_.size())
+        return true;
+    bool need = chunk_tile.size() > 1 && !(flags_[chunk_tile[0]] & kStartsRow);
+    for (size_t c = 1; !need && c + 1 < chunk_tile.size(); ++c) {
+        const uint32_t t = chunk_tile[c];
+        const uint8_t f = flags_[t];
+        const bool looked = (f & kMultiSeg) && (flags_[t - 1] & (kStartsRow | kMultiSeg));
+        need = !(f & kStartsRow) && !looked;
+    }
+    return need;
+}
+
 int detail::device_cus() {
     static int cus[64] = {};
     int dev = 0;
@@ -243,17 +257,46 @@ void fill_params(const DevFormat& A, Scratch& S, bool f32, int k, const void* x,
 
 } // namespace
 
+uint64_t detail::resident_tile_waves(uint32_t sigma, bool codes, bool dot) {
+    int b = 2;
+    switch (sigma) {
+#define DIM_WAVES_CASE(S)                                                                                             \
+    case S:                                                                                                           \
+        if constexpr (coded_kernel<1, S>())                                                                           \
+            if (codes) {                                                                                              \
+                b = dot ? tile_blocks_per_cu<double, S, 1, true, true>() : tile_blocks_per_cu<double, S, 1, false, true>(); \
+                break;                                                                                                \
+            }                                                                                                         \
+        b = dot ? tile_blocks_per_cu<double, S, 1, true, false>() : tile_blocks_per_cu<double, S, 1, false, false>();  \
+        break;
+        DIM_FOR_EACH_SIGMA(DIM_WAVES_CASE)
+#undef DIM_WAVES_CASE
+        default: break;
+    }
+    return uint64_t(device_cus()) * b * 4;
+}
+
+TileParams detail::make_product_params(const DevFormat& A, Scratch& S, const void* x, void* y) {
+    TileParams tp;
+    FixParams fp;
+    fill_params(A, S, false, 1, x, y, 0, A.tiles, 0, 0, A.n_empty, nullptr, tp, fp);
+    tp.chunk_tile = A.chunk_tile;
+    tp.chunk_sys = A.chunk_sys;
+    tp.nchunks = A.n_chunk_tab;
+    tp.tpw = 0;
+    tp.lookahead = 1;
+    return tp;
+}
+
 hipError_t launch_spmv(const DevFormat& A, Scratch& S, bool f32, int k, const void* x, void* y, uint32_t tile_begin,
                        uint32_t tile_end, uint32_t row_base, uint32_t empty_begin, uint32_t empty_end, bool dot,
                        const int* done, hipStream_t stream, uint32_t* dot_blocks, uint64_t ldx, uint64_t ldy,
-                       const double** dot_parts, const uint32_t* tile_sys, const int* sys_done) {
+                       const double** dot_parts, const int* sys_done) {
     TileParams tp;
     FixParams fp;
     fill_params(A, S, f32, k, x, y, tile_begin, tile_end, row_base, empty_begin, empty_end, done, tp, fp, ldx, ldy);
-    if (tile_sys && sys_done && k == 1 && !dot) {
-        tp.tile_sys = tile_sys;
+    if (k == 1)
         tp.sys_done = sys_done;
-    }
     uint32_t nb = 0;
     const double* parts = S.dot_blk;
     if (k == 1) {

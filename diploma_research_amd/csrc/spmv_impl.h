@@ -199,10 +199,17 @@ struct TileParams {
     uint32_t empty_begin, empty_end; // empty rows written by this kernel
     int k;
     int lookahead;
-    // Batched solves (single-vector products without a dot): the system of
-    // each tile and the systems' stop flags, or null. Each system owns whole
+    // Chunk table of a block-diagonal batch (null: chunks of tpw tiles from
+    // tile_begin): chunk c covers tiles [chunk_tile[c], chunk_tile[c + 1]) of
+    // system chunk_sys[c], so no chunk holds tiles of two systems. The
+    // dictionary of a coded chunk is then col_dict + kMaxDiagonals *
+    // chunk_sys[c] (every system keeps its own diagonals), and a dot product
+    // leaves one partial per chunk in dot_chunk[c]. Each system owns whole
     // tiles, so no row crosses from one system into another.
-    const uint32_t* tile_sys;
+    const uint32_t* chunk_tile;
+    const uint32_t* chunk_sys;
+    // the systems' stop flags (single-vector products with a chunk table; may
+    // be null): the chunks of a system whose flag is set are skipped
     const int* sys_done;
     // x and y hold k columns: element (i, j) of x at i*rs + j*xcs, of y at
     // i*rs + j*ycs (row-major n x K blocks: rs = K, xcs = ycs = 1;
@@ -219,41 +226,40 @@ constexpr int smm_group() {
 }
 constexpr int kSmmRegSigma = 16;
 
+// One wavefront's work on chunk c: the product over the chunk's tiles, its
+// carries and look-ahead, its share of the empty rows, and for CG the dot
+// partial of the rows it writes. Returns that partial summed over the
+// wavefront for single-vector products with DOT (multi-RHS products store
+// theirs per chunk); the caller stores it. `stopped` (single-vector products
+// with a chunk table): the chunk's system has stopped, so its rows are left
+// as they are (a stopped system's product is not used); the chunk still
+// writes its flags, as a chunk that starts a row and takes no carry, and its
+// share of the empty rows. The fix-up kernel may then add stale head and
+// tail slots of such a system's chunks into its rows of y; nothing reads
+// them.
 // CODES: the matrix carries column codes (every tile clean, sigma <= 8); the
 // kernel then streams one code word per lane instead of sigma column indices.
 template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES = false>
-__global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
-    if (p.done && *p.done)
-        return;
+__device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t c, const bool stopped) {
     const int lane = threadIdx.x & 63;
-    // the chunk index is uniform across the wavefront: keep it (and every
-    // per-tile quantity derived from it) in scalar registers
-    const uint32_t c = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
-    if (c >= p.nchunks)
-        return; // wave-uniform
-    const uint32_t t0 = p.tile_begin + c * p.tpw;
-    const uint32_t t1 = min(t0 + p.tpw, p.tile_end);
-    if constexpr (KMAX == 1 && !DOT) {
-        // A chunk whose tiles all belong to systems that have stopped leaves
-        // their rows as they are (a stopped system's product is not used).
-        // It still writes its flags, as a chunk that starts a row and takes
-        // no carry, and its share of the empty rows.
-        if (p.tile_sys) {
-            const uint32_t s0 = sload(p.tile_sys + t0), s1 = sload(p.tile_sys + t1 - 1);
-            bool stopped = true;
-            for (uint32_t q = s0; q <= s1 && stopped; ++q)
-                stopped = sload(p.sys_done + q) != 0;
-            if (stopped) {
-                if (threadIdx.x % 64 == 0)
-                    p.chunk_flags[c] = kChunkStarts | kChunkRowStart;
-                if (p.lookahead) {
-                    V* __restrict__ yy = static_cast<V*>(p.y);
-                    for (uint32_t i = p.empty_begin + c * 64 + threadIdx.x % 64; i < p.empty_end;
-                         i += p.nchunks * 64)
-                        yy[static_cast<size_t>(p.empty_rows[i] - p.row_base)] = V(0);
-                }
-                return; // wave-uniform
+    uint32_t t0, t1;
+    if (p.chunk_tile) {
+        t0 = sload(p.chunk_tile + c);
+        t1 = sload(p.chunk_tile + c + 1);
+    } else {
+        t0 = p.tile_begin + c * p.tpw;
+        t1 = min(t0 + p.tpw, p.tile_end);
+    }
+    if constexpr (KMAX == 1) {
+        if (stopped) {
+            if (lane == 0)
+                p.chunk_flags[c] = kChunkStarts | kChunkRowStart;
+            if (p.lookahead) {
+                V* __restrict__ yy = static_cast<V*>(p.y);
+                for (uint32_t i = p.empty_begin + c * 64 + lane; i < p.empty_end; i += p.nchunks * 64)
+                    yy[static_cast<size_t>(p.empty_rows[i] - p.row_base)] = V(0);
             }
+            return 0.0; // wave-uniform
         }
     }
 
@@ -311,8 +317,10 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     constexpr int NX = XREG ? SIGMA : 1;
     const V* __restrict__ vals_all = static_cast<const V*>(p.vals);
     const W* __restrict__ bits_all = static_cast<const W*>(p.bit_flag);
-    // the diagonals of a coded matrix, lane d holding diagonal d
-    const int dtab = CODED && lane < 16 ? p.col_dict[lane] : 0;
+    // the diagonals of a coded matrix (of the chunk's system in a batch),
+    // lane d holding diagonal d
+    const int dtab =
+      CODED && lane < 16 ? p.col_dict[(p.chunk_sys ? sload(p.chunk_sys + c) * kMaxDiagonals : 0u) + lane] : 0;
 
     struct Stage {
         V v[NREG];
@@ -421,7 +429,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 const int ls = __ffsll(static_cast<long long>(any)) - 1;
                 const uint32_t bl = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wb_end), ls));
                 const int rs = __ffs(static_cast<int>(bl)) - 1;
-                constexpr uint32_t full = SIGMA >= 32 ? ~0u : ((1u << SIGMA) - 1u);
+                constexpr uint32_t full = static_cast<uint32_t>((uint64_t{1} << SIGMA) - 1u);
                 m_end = lane < ls ? full : (lane == ls ? ((1u << rs) - 1u) : 0u);
             }
         }
@@ -940,10 +948,44 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             y[static_cast<size_t>(p.empty_rows[i] - p.row_base)] = V(0);
     }
     if constexpr (DOT && KMAX == 1) {
+        return wave_sum(dacc[0]);
+    } else if constexpr (DOT) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            const double s = wave_sum(dacc[j]);
+            if (lane == 0 && j < k)
+                p.dot_chunk[static_cast<size_t>(c) * k + j] = s;
+        }
+    }
+    return 0.0;
+}
+
+template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES = false>
+__global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
+    if (p.done && *p.done)
+        return;
+    // the chunk index is uniform across the wavefront: keep it (and every
+    // per-tile quantity derived from it) in scalar registers
+    const uint32_t c = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+    if (c >= p.nchunks)
+        return; // wave-uniform
+    bool stopped = false;
+    if constexpr (KMAX == 1) {
+        if (p.chunk_sys && p.sys_done)
+            stopped = sload(p.sys_done + sload(p.chunk_sys + c)) != 0;
+    }
+    const double s = csr5_chunk<V, SIGMA, KMAX, DOT, CODES>(p, c, stopped);
+    if constexpr (DOT && KMAX == 1) {
+        const int lane = threadIdx.x & 63;
+        if (p.chunk_tile) {
+            // one partial per chunk: a batch sums them per system
+            if (lane == 0)
+                p.dot_chunk[c] = s;
+            return;
+        }
         // One partial per block of four chunks (block b: chunks 4b..4b+3, in
         // order), one per chunk in a last block that holds fewer than four:
         // dot_partials() of them.
-        const double s = wave_sum(dacc[0]);
         const uint32_t base = c & ~3u;
         if (base + 4 <= p.nchunks) { // block-uniform: every wave of the block is here
             __shared__ double red[4];
@@ -954,13 +996,6 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
                 p.dot_chunk[base >> 2] = (red[0] + red[1]) + (red[2] + red[3]);
         } else if (lane == 0) {
             p.dot_chunk[(base >> 2) + (c & 3u)] = s;
-        }
-    } else if constexpr (DOT) {
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j) {
-            const double s = wave_sum(dacc[j]);
-            if (lane == 0 && j < k)
-                p.dot_chunk[static_cast<size_t>(c) * k + j] = s;
         }
     }
 }
@@ -985,7 +1020,12 @@ struct FixParams {
     const void* tail;
     const double* dot_chunk;
     double* dot_blk;
+    // chunk-table products with a single-vector dot: the dot partials stay
+    // one per chunk, and the chunk that finishes a row adds that row's term
+    // to its own partial here (null: partials are reduced into dot_blk)
+    double* dot_rw;
     const int* done;
+    const uint32_t* chunk_tile; // as TileParams::chunk_tile
     uint32_t tile_begin, row_base, empty_begin, empty_end;
     uint32_t tpw, nchunks;
     uint32_t ndot;               // dot partials in dot_chunk
@@ -1011,6 +1051,7 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
     const size_t ycs = KMAX == 1 ? 0 : p.ycs;
     const V* head = static_cast<const V*>(p.head);
     const V* tail = static_cast<const V*>(p.tail);
+    const bool per_chunk = DOT && KMAX == 1 && p.dot_rw; // block-uniform
     double dacc[KMAX];
 #pragma unroll
     for (int j = 0; j < KMAX; ++j)
@@ -1024,7 +1065,7 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
             while (kk >= 0 && !(p.chunk_flags[kk] & kChunkRowStart))
                 --kk;
             const uint32_t from = kk >= 0 ? static_cast<uint32_t>(kk) : 0u;
-            const uint32_t row = p.tile_ptr[p.tile_begin + c * p.tpw] & ~kDirty;
+            const uint32_t row = p.tile_ptr[p.chunk_tile ? p.chunk_tile[c] : p.tile_begin + c * p.tpw] & ~kDirty;
             V* yr = y + static_cast<size_t>(row - p.row_base) * rs;
 #pragma unroll
             for (int j = 0; j < KMAX; ++j) {
@@ -1039,14 +1080,20 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
                         s = head[static_cast<size_t>(c) * k + j];
                     }
                     yr[j * ycs] = s;
-                    if constexpr (DOT)
-                        dacc[j] += static_cast<double>(x[static_cast<size_t>(row) * rs + j * xcs]) * static_cast<double>(s);
+                    if constexpr (DOT) {
+                        const double term =
+                          static_cast<double>(x[static_cast<size_t>(row) * rs + j * xcs]) * static_cast<double>(s);
+                        if (per_chunk)
+                            p.dot_rw[c] += term;
+                        else
+                            dacc[j] += term;
+                    }
                 }
             }
         }
     }
     if constexpr (DOT) {
-        if (c < p.ndot) {
+        if (!per_chunk && c < p.ndot) {
 #pragma unroll
             for (int j = 0; j < KMAX; ++j)
                 if (j < k)
@@ -1063,6 +1110,8 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
     }
 
     if constexpr (DOT) {
+        if (per_chunk)
+            return;
         __shared__ double red[4][KMAX];
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -1087,7 +1136,7 @@ struct Plan {
 int device_cus();
 
 template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES>
-Plan make_plan(uint32_t ntiles) {
+int tile_blocks_per_cu() {
     static int blocks_per_cu = 0;
     if (!blocks_per_cu) {
         int b = 0;
@@ -1097,6 +1146,12 @@ Plan make_plan(uint32_t ntiles) {
             b = 2;
         blocks_per_cu = b;
     }
+    return blocks_per_cu;
+}
+
+template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES>
+Plan make_plan(uint32_t ntiles) {
+    const int blocks_per_cu = tile_blocks_per_cu<V, SIGMA, KMAX, DOT, CODES>();
     Plan pl;
     if (!ntiles)
         return pl;
@@ -1159,11 +1214,26 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
         *parts = fp.dot_blk;
     const uint32_t ntiles = tp.tile_end - tp.tile_begin;
     const bool codes = A.col_code != nullptr;
-    const Plan pl = dot ? plan_by_sigma<V, KMAX, true>(A.sigma, ntiles, codes)
-                        : plan_by_sigma<V, KMAX, false>(A.sigma, ntiles, codes);
+    // a batch's products over the whole matrix walk its chunk table (chunks
+    // that never hold two systems)
+    const bool table = A.chunk_tile && tp.tile_begin == 0 && tp.tile_end == A.tiles;
+    Plan pl = dot ? plan_by_sigma<V, KMAX, true>(A.sigma, ntiles, codes)
+                  : plan_by_sigma<V, KMAX, false>(A.sigma, ntiles, codes);
+    if (table) {
+        pl.tpw = 0;
+        pl.nchunks = A.n_chunk_tab;
+        pl.tgrid = (pl.nchunks + 3) / 4;
+        pl.fgrid = (pl.nchunks + 255) / 256;
+        tp.chunk_tile = fp.chunk_tile = A.chunk_tile;
+        tp.chunk_sys = A.chunk_sys;
+    } else {
+        tp.chunk_tile = fp.chunk_tile = nullptr;
+        tp.chunk_sys = nullptr;
+        tp.sys_done = nullptr;
+    }
     tp.tpw = fp.tpw = pl.tpw;
     tp.nchunks = fp.nchunks = pl.nchunks;
-    fp.ndot = dot_partials(KMAX, pl.nchunks);
+    fp.ndot = table ? pl.nchunks : dot_partials(KMAX, pl.nchunks);
     // Single-vector products finish rows across chunk boundaries by
     // look-ahead and write the empty rows themselves; the fix-up kernel then
     // runs only when some boundary needs it (with a dot, the chunks' partials
@@ -1172,7 +1242,9 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
     if (KMAX == 1 && pl.tgrid) {
         tp.lookahead = 1;
         fp.empty_end = fp.empty_begin;
-        fixup = !A.binfo || A.binfo->needs_fixup(A.binfo_base + tp.tile_begin, A.binfo_base + tp.tile_end, pl.tpw);
+        fixup = table ? A.chunk_tab_fixup
+                      : !A.binfo ||
+                          A.binfo->needs_fixup(A.binfo_base + tp.tile_begin, A.binfo_base + tp.tile_end, pl.tpw);
     } else {
         tp.lookahead = 0;
         tp.empty_end = tp.empty_begin;
@@ -1192,12 +1264,28 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
             *parts = tp.dot_chunk;
         return fp.ndot;
     }
+    const bool per_chunk = table && dot && KMAX == 1;
+    fp.dot_rw = per_chunk ? tp.dot_chunk : nullptr;
     if (dot)
         hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, true>), dim3(pl.fgrid), dim3(256), 0, st, fp);
     else
         hipLaunchKernelGGL((k_csr5_fixup<V, KMAX, false>), dim3(pl.fgrid), dim3(256), 0, st, fp);
+    if (per_chunk) {
+        if (parts)
+            *parts = tp.dot_chunk;
+        return fp.ndot;
+    }
     return pl.fgrid;
 }
+
+// Wavefronts of the FP64 single-vector tile kernel (with or without the dot
+// epilogue) that the device keeps resident for a format of tile height sigma.
+uint64_t resident_tile_waves(uint32_t sigma, bool codes, bool dot);
+
+// Parameters of the single-vector product y = A x with p.q-style dot
+// partials over a batch's whole chunk table (look-ahead on, empty rows
+// written), for kernels that call csr5_chunk themselves.
+TileParams make_product_params(const DevFormat& A, Scratch& S, const void* x, void* y);
 
 // Multi-RHS launchers, one translation unit per k bound (spmv_k*.hip).
 uint32_t launch_pair_k2(bool f32, const DevFormat& A, const TileParams& tp, const FixParams& fp, bool dot,

@@ -28,6 +28,7 @@ DIM_CSR5_PLAIN_COLUMNS = 4
 DIM_CG_F64 = 0
 DIM_CG_F32_REFINED = 1
 DIM_CG_TRANSPOSE = 1
+DIM_CG_PERSISTENT = 2
 DIM_CG_NOALLOC_MAX_ROWS = 2097152
 
 # Every symbol include/dim_gpu.h declares (checked by the CPU test suite).

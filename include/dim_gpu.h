@@ -174,9 +174,14 @@ typedef struct dim_cg_opts {
     uint32_t check_every;  /* iterations between host convergence checks (0 = 16) */
     double refine_delta;   /* F32_REFINED: FP64 residual replacement when ||r|| drops by this factor (0 = 0.1) */
     double* history;       /* optional HOST array of max_iters+1: ||r_i||/||b|| (recursive residual) */
-    uint32_t flags;        /* DIM_CG_TRANSPOSE: iterate with A^T (the adjoint solve A^T l = g); batched only */
+    uint32_t flags;        /* DIM_CG_* below; batched only */
 } dim_cg_opts;
+/* iterate with A^T (the adjoint solve A^T l = g) */
 #define DIM_CG_TRANSPOSE 1u
+/* run the whole iteration loop in one persistent cooperative kernel with
+ * grid barriers instead of replaying a captured launch sequence (not with
+ * DIM_CG_TRANSPOSE; DIM_UNSUPPORTED when the kernel cannot be resident) */
+#define DIM_CG_PERSISTENT 2u
 
 typedef struct dim_cg_result {
     uint32_t iterations;
