@@ -22,7 +22,7 @@ matrix and vectors (625 MB). The same JSON line carries:
     with its error against the FP64 oracle.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-extras] [--extras cg,parts,...]
-    python bench.py --traffic-sweep   # the products rocprofv3 counter runs profile
+    python bench.py --traffic-sweep [powerlaw]   # the products rocprofv3 counter runs profile
 """
 from __future__ import annotations
 
@@ -460,31 +460,37 @@ def run_cpu_leg(with_c4):
 
 # -------------------------------------------------------- traffic sweep --
 
-def traffic_sweep(copies=3, reps=6):
+def traffic_sweep(which="all", copies=3, reps=6):
     """The products that the rocprofv3 counter runs profile: the 128^3
     Poisson SpMV at every tile height (column-coded and 32-bit-column kernels
     where both exist), `reps` launches each rotating over `copies` copies,
-    then the power-law product in its autotuned form, and a 256 MiB device
-    copy of known size that scales the counters."""
+    then the power-law product (autotuned, or with which="powerlaw" alone in
+    the column-sliced form at the fixed tile height 4, so that every counter
+    pass profiles the same kernel), and a 256 MiB device copy of known size
+    that scales the counters."""
     import torch
 
     from diploma_research_amd import Csr5, matgen
 
-    n = 128 ** 3
-    rp, ci, v = matgen.poisson3d(128)
-    x = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, n)).cuda()
-    for sigma in (4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32):
-        for plain in ((False, True) if sigma <= 8 else (False,)):
-            mats = [Csr5.from_csr((n, n, rp, ci, v), sigma=sigma, plain_columns=plain) for _ in range(copies)]
-            xs = [x.clone() for _ in mats]
-            ys = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in mats]
-            time_spmv(mats, xs, ys, reps, 0)
-            log(f"sigma {sigma} {'plain' if plain else 'coded' if mats[0].info.column_codes else ''} done")
-            del mats, xs, ys
-    del rp, ci, v
+    if which == "all":
+        n = 128 ** 3
+        rp, ci, v = matgen.poisson3d(128)
+        x = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, n)).cuda()
+        for sigma in (4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32):
+            for plain in ((False, True) if sigma <= 8 else (False,)):
+                mats = [Csr5.from_csr((n, n, rp, ci, v), sigma=sigma, plain_columns=plain) for _ in range(copies)]
+                xs = [x.clone() for _ in mats]
+                ys = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in mats]
+                time_spmv(mats, xs, ys, reps, 0)
+                log(f"sigma {sigma} {'plain' if plain else 'coded' if mats[0].info.column_codes else ''} done")
+                del mats, xs, ys
+        del rp, ci, v
     m = 4_000_000
     rp, ci, v = matgen.powerlaw(m, seed=2024)
-    A = Csr5.from_csr((m, m, rp, ci, v), sigma=0, keep_f32=True)
+    if which == "powerlaw":
+        A = Csr5.from_csr((m, m, rp, ci, v), sigma=4, xcd_slices=True)
+    else:
+        A = Csr5.from_csr((m, m, rp, ci, v), sigma=0, keep_f32=True)
     log(f"power-law: sigma {A.sigma}, slices {A.info.xcd_slices} (stacked sigma {A.info.xcd_sigma})")
     xp = torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, m)).cuda()
     yp = torch.empty(m, dtype=torch.float64, device="cuda")
@@ -514,8 +520,9 @@ def main():
                     help="skip the oracle's CG solve of the 192^3 system (about a minute of CPU time)")
     ap.add_argument("--fd-entries", type=int, default=64,
                     help="stored entries the batched adjoint is checked on by finite differences (0: none)")
-    ap.add_argument("--traffic-sweep", action="store_true",
-                    help="run the products the rocprofv3 counter runs profile, and nothing else")
+    ap.add_argument("--traffic-sweep", nargs="?", const="all", choices=("all", "powerlaw"),
+                    help="run the products the rocprofv3 counter runs profile, and nothing else "
+                         "(powerlaw: only the power-law product, column-sliced at tile height 4)")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.cpu_leg:
@@ -533,7 +540,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if a.traffic_sweep:
-        traffic_sweep()
+        traffic_sweep(a.traffic_sweep)
         return
     side = 128
     n = side ** 3
