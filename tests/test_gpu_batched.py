@@ -208,6 +208,68 @@ def test_adjoint_gradient_matches_dense_reference_and_finite_differences(cuda, s
     assert worst <= 1e-6, worst
 
 
+def test_adjoint_gradient_second_order_fd_h1e6_through_gpu_solves(cuda):
+    """The adjoint's dL/dvals against second-order central differences with
+    h = 1e-6 on 64 random stored entries, every evaluation a GPU CG solve
+    (rtol 1e-15). [L(+h) - L(-h)] / 2h is evaluated exactly as
+    -0.5 d.(x(+h) + x(-h)) with A(+h) d = E x(-h) (E the unit perturbation of
+    a_ij and a_ji), the resolvent identity x(+h) - x(-h) = -2h A(+h)^-1 E
+    x(-h): the solves' rounding is then not divided by h, and what is left
+    is the O(h^2) truncation. Bound: 1e-6 relative on every entry."""
+    import torch
+
+    from diploma_research_amd.autograd import solve
+
+    systems = _systems([20, 35, 28, 41], 0.5, seed=41)
+    A = Csr5.from_csr_batched(systems, sigma=0)
+    rng = np.random.default_rng(19)
+    bs = [rng.uniform(-1, 1, n) for n, *_ in systems]
+    vals_np = np.concatenate([v for *_, v in systems])
+    vals = torch.from_numpy(vals_np.copy()).cuda().requires_grad_(True)
+    b = torch.from_numpy(np.concatenate(bs)).cuda()
+    x = solve(A, vals, b, rtol=1e-14, max_iters=20000)
+    (0.5 * (x * x).sum()).backward()
+    gv = vals.grad.cpu().numpy()
+
+    h = 1e-6
+    starts = np.concatenate([[0], np.cumsum([len(v) for *_, v in systems])])
+    singles = {}
+    worst = 0.0
+    for e in rng.choice(len(vals_np), size=64, replace=False):
+        s = int(np.searchsorted(starts, e, side="right") - 1)
+        n, rp, ci, v = systems[s]
+        if s not in singles:
+            singles[s] = Csr5.from_csr((n, n, rp, ci, v), sigma=A.sigma)
+        M = singles[s]
+        k = int(e - starts[s])
+        i = int(np.searchsorted(rp.astype(np.int64), k, side="right") - 1)
+        j = int(ci[k])
+        mirror = [k]
+        if i != j:
+            row_j = slice(int(rp[j]), int(rp[j + 1]))
+            mirror.append(int(rp[j]) + int(np.nonzero(ci[row_j] == i)[0][0]))
+        bd = to_dev(bs[s])
+        xs = {}
+        for step in (-h, h):
+            vv = v.copy()
+            for m in mirror:
+                vv[m] += step
+            M.set_values(torch.from_numpy(vv).cuda())
+            xs[step], r = conjugate_gradient(M, bd, rtol=1e-15, max_iters=20000)
+            assert r.converged
+        w = torch.zeros_like(bd)
+        if i != j:
+            w[i], w[j] = xs[-h][j], xs[-h][i]
+        else:
+            w[i] = xs[-h][i]
+        d, r = conjugate_gradient(M, w, rtol=1e-15, max_iters=20000)  # M holds A(+h)
+        assert r.converged
+        fd = -0.5 * float(torch.dot(d, xs[h] + xs[-h]))
+        an = sum(gv[starts[s] + m] for m in mirror)
+        worst = max(worst, abs(fd - an) / max(abs(an), 1e-300))
+    assert worst <= 1e-6, worst
+
+
 def test_adjoint_gradient_wrt_b_by_finite_differences_on_gpu(cuda):
     """FD through the GPU solver itself (every evaluation a batched CG)."""
     import torch
