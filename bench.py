@@ -22,7 +22,7 @@ matrix and vectors (625 MB). The same JSON line carries:
     with its error against the FP64 oracle.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-extras] [--extras cg,parts,...]
-    python bench.py --traffic-sweep [powerlaw]   # the products rocprofv3 counter runs profile
+    python bench.py --traffic-sweep [powerlaw|spmm]   # the products rocprofv3 counter runs profile
 """
 from __future__ import annotations
 
@@ -489,11 +489,32 @@ def traffic_sweep(which="all", copies=3, reps=6):
     then the power-law product (autotuned, or with which="powerlaw" alone in
     the column-sliced form at the fixed tile height 4, so that every counter
     pass profiles the same kernel), and a 256 MiB device copy of known size
-    that scales the counters."""
+    that scales the counters. which="spmm" runs only the multi-RHS products
+    of the 192^3 variable-coefficient operator at k = 8, 2 and 1."""
     import torch
 
     from diploma_research_amd import Csr5, matgen
 
+    if which == "spmm":
+        # the multi-RHS product of the 192^3 variable-coefficient operator
+        # (column-major blocks, tile height 7, column codes) at k = 8, 2, 1
+        side = 192
+        n = side ** 3
+        rp, ci, v = matgen.varcoef((side, side, side), 1.0, seed=4)
+        A = Csr5.from_csr((n, n, rp, ci, v), sigma=7)
+        del rp, ci, v
+        for k in (8, 2, 1):
+            X = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, (k, n))).cuda().t()
+            Y = torch.empty((k, n), dtype=torch.float64, device="cuda").t()
+            for _ in range(reps + 2):
+                if k > 1:
+                    A.spmm(X, Y)
+                else:
+                    A.spmv(X[:, 0].contiguous(), Y[:, 0])
+            torch.cuda.synchronize()
+            log(f"spmm k={k} done")
+            del X, Y
+        return
     if which == "all":
         n = 128 ** 3
         rp, ci, v = matgen.poisson3d(128)
@@ -542,9 +563,10 @@ def main():
                     help="skip the oracle's CG solve of the 192^3 system (about a minute of CPU time)")
     ap.add_argument("--fd-entries", type=int, default=64,
                     help="stored entries the batched adjoint is checked on by finite differences (0: none)")
-    ap.add_argument("--traffic-sweep", nargs="?", const="all", choices=("all", "powerlaw"),
+    ap.add_argument("--traffic-sweep", nargs="?", const="all", choices=("all", "powerlaw", "spmm"),
                     help="run the products the rocprofv3 counter runs profile, and nothing else "
-                         "(powerlaw: only the power-law product, column-sliced at tile height 4)")
+                         "(powerlaw: only the power-law product, column-sliced at tile height 4; "
+                         "spmm: the 192^3 multi-RHS products at k = 8, 2, 1)")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.cpu_leg:

@@ -103,6 +103,7 @@ dim_status upload_format(const HostFormat& f, bool keep_f32, dim_csr5** out, boo
     DIM_HIP_TRY(dev_alloc(h.get(), s.chunk_flags, T));
     DIM_HIP_TRY(dev_alloc(h.get(), s.dot_chunk, T * kMaxK));
     DIM_HIP_TRY(dev_alloc(h.get(), s.dot_blk, ((T + 255) / 256) * kMaxK));
+    DIM_HIP_TRY(upload(h.get(), s.queue, std::vector<uint32_t>(kQueueWords, 0u)));
     h->sys_row = {0, f.rows};
     h->sys_tile = {0, f.tiles};
     h->sys_nnz_csr = {0, f.nnz};

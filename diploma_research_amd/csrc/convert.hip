@@ -314,6 +314,7 @@ dim_status convert(uint32_t rows, uint32_t cols, uint64_t nnz, const uint32_t* r
     DIM_HIP_TRY(dev_alloc(h.get(), sc.chunk_flags, Tm));
     DIM_HIP_TRY(dev_alloc(h.get(), sc.dot_chunk, Tm * kMaxK));
     DIM_HIP_TRY(dev_alloc(h.get(), sc.dot_blk, ((Tm + 255) / 256) * kMaxK));
+    DIM_HIP_TRY(upload(h.get(), sc.queue, std::vector<uint32_t>(kQueueWords, 0u)));
     DIM_HIP_TRY(hipStreamSynchronize(s));
     // host copy of the tile flags for the launch planner
     std::vector<uint8_t> hflags(size_t{T} + 1);

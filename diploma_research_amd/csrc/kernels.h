@@ -66,6 +66,13 @@ struct DevFormat {
     bool chunk_tab_fixup = true;
 };
 
+// The multi-RHS products' ordered sweep (TileParams::queue): one counter
+// pair per XCD, 128 bytes apart, zero between launches (the last wavefront
+// of each range clears its pair).
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kQueueStride = 32;
+constexpr uint32_t kQueueWords = kXcds * kQueueStride;
+
 // Per-matrix scratch: tile carries and reduction partials (the GPU
 // counterpart of the reference's calibrator, include/dim/mat/storage_formats/
 // csr5/calibrator.h:14-31). Sized for up to kMaxK right-hand sides.
@@ -76,6 +83,7 @@ struct Scratch {
     uint8_t* chunk_flags = nullptr; // chunks
     double* dot_chunk = nullptr;   // chunks * kMaxK
     double* dot_blk = nullptr;     // ceil(chunks/256) * kMaxK
+    uint32_t* queue = nullptr;     // kQueueWords, zero between products
 };
 
 // y = A x over tiles [tile_begin, tile_end). x and y are n x k, row-major.

@@ -215,7 +215,15 @@ struct TileParams {
     // i*rs + j*ycs (row-major n x K blocks: rs = K, xcs = ycs = 1;
     // column-major: rs = 1, xcs and ycs the leading dimensions)
     uint64_t rs, xcs, ycs;
+    // Ordered sweep (multi-RHS products; null: one chunk per wavefront): the
+    // chunks are cut in kXcds contiguous ranges, and the wavefronts of blocks
+    // b with b % kXcds == x take the chunks of range x in order from the
+    // counter queue[x * kQueueStride] (its count of finished wavefronts at
+    // queue[x * kQueueStride + 1]), so that the wavefronts resident on one
+    // XCD work on neighbouring chunks at any time.
+    uint32_t* queue;
 };
+
 
 // Multi-RHS products: right-hand sides whose gathers are in flight together
 // in the fast path, and the tallest tile whose matrix column a lane keeps in
@@ -964,6 +972,48 @@ template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES = false>
 __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     if (p.done && *p.done)
         return;
+    if constexpr (KMAX > 1) {
+        // One chunk per wavefront, or (queue set) the ordered sweep. Blocks
+        // are dispatched to the XCDs in turn (block b on XCD b % kXcds): in
+        // the sweep the wavefronts of one XCD share one range of chunks and
+        // its counter, taking the chunks in order, so that their x gathers
+        // and the rows they write stay within a window its L2 holds. Every
+        // chunk is taken once; the wavefront that finishes last clears the
+        // pair. Both forms share one loop, so the chunk code is emitted once.
+        // Every lane runs the counter's atomic (lane 0 adds 1, the others 0),
+        // so that the chunk index never comes out of a lane-0-only branch: a
+        // loop whose exit followed a value produced under such a branch was
+        // compiled with a per-lane exit that lanes 1..63 never took.
+        const int lane = threadIdx.x & 63;
+        const bool sweep = p.queue != nullptr;
+        const uint32_t xr = blockIdx.x % kXcds;
+        uint32_t* q = sweep ? p.queue + xr * kQueueStride : nullptr;
+        uint32_t c0 = 0, c1 = p.nchunks, c;
+        if (sweep) {
+            const uint32_t per = (p.nchunks + kXcds - 1) / kXcds;
+            c0 = min(xr * per, p.nchunks);
+            c1 = min(c0 + per, p.nchunks);
+            c = c0 + __builtin_amdgcn_readfirstlane(atomicAdd(q, lane == 0 ? 1u : 0u));
+        } else {
+            c = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+        }
+        while (c < c1) { // wave-uniform
+            (void)csr5_chunk<V, SIGMA, KMAX, DOT, CODES>(p, c, false);
+            if (!sweep)
+                break;
+            c = c0 + __builtin_amdgcn_readfirstlane(atomicAdd(q, lane == 0 ? 1u : 0u));
+        }
+        if (sweep) {
+            // wavefronts of this range: 4 per block b < gridDim.x with
+            // b % kXcds == xr
+            const uint32_t waves = 4u * (gridDim.x / kXcds + (xr < gridDim.x % kXcds ? 1u : 0u));
+            if (lane == 0 && atomicAdd(q + 1, 1u) + 1u == waves) {
+                atomicExch(q, 0u);
+                atomicExch(q + 1, 0u);
+            }
+        }
+        return;
+    }
     // the chunk index is uniform across the wavefront: keep it (and every
     // per-tile quantity derived from it) in scalar registers
     const uint32_t c = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
@@ -1131,6 +1181,7 @@ __global__ __launch_bounds__(256) void k_csr5_fixup(FixParams p) {
 // keeps resident for this kernel.
 struct Plan {
     uint32_t tpw = 1, nchunks = 0, tgrid = 0, fgrid = 1;
+    bool sweep = false; // multi-RHS ordered sweep (TileParams::queue)
 };
 
 int device_cus();
@@ -1149,6 +1200,9 @@ int tile_blocks_per_cu() {
     return blocks_per_cu;
 }
 
+// Tiles per chunk of the multi-RHS products' ordered sweep.
+uint32_t sweep_tiles();
+
 template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES>
 Plan make_plan(uint32_t ntiles) {
     const int blocks_per_cu = tile_blocks_per_cu<V, SIGMA, KMAX, DOT, CODES>();
@@ -1156,6 +1210,15 @@ Plan make_plan(uint32_t ntiles) {
     if (!ntiles)
         return pl;
     const uint64_t waves = static_cast<uint64_t>(device_cus()) * blocks_per_cu * 4;
+    if (KMAX > 1 && sweep_tiles()) {
+        // ordered sweep: short chunks, every resident wavefront looping
+        pl.sweep = true;
+        pl.tpw = sweep_tiles();
+        pl.nchunks = (ntiles + pl.tpw - 1) / pl.tpw;
+        pl.tgrid = static_cast<uint32_t>(std::max<uint64_t>(kXcds, waves / 4));
+        pl.fgrid = (pl.nchunks + 255) / 256;
+        return pl;
+    }
     pl.tpw = static_cast<uint32_t>((ntiles + waves - 1) / waves);
     pl.nchunks = (ntiles + pl.tpw - 1) / pl.tpw;
     pl.tgrid = (pl.nchunks + 3) / 4;
@@ -1220,6 +1283,7 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
     Plan pl = dot ? plan_by_sigma<V, KMAX, true>(A.sigma, ntiles, codes)
                   : plan_by_sigma<V, KMAX, false>(A.sigma, ntiles, codes);
     if (table) {
+        pl.sweep = false;
         pl.tpw = 0;
         pl.nchunks = A.n_chunk_tab;
         pl.tgrid = (pl.nchunks + 3) / 4;
@@ -1233,6 +1297,8 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
     }
     tp.tpw = fp.tpw = pl.tpw;
     tp.nchunks = fp.nchunks = pl.nchunks;
+    if (!pl.sweep)
+        tp.queue = nullptr;
     fp.ndot = table ? pl.nchunks : dot_partials(KMAX, pl.nchunks);
     // Single-vector products finish rows across chunk boundaries by
     // look-ahead and write the empty rows themselves; the fix-up kernel then
