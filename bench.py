@@ -22,7 +22,7 @@ matrix and vectors (625 MB). The same JSON line carries:
     with its error against the FP64 oracle.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-extras] [--extras cg,parts,...]
-    python bench.py --traffic-sweep [powerlaw|spmm]   # the products rocprofv3 counter runs profile
+    python bench.py --traffic-sweep [powerlaw|spmm|spmm-all]   # the products rocprofv3 counter runs profile
 """
 from __future__ import annotations
 
@@ -490,30 +490,35 @@ def traffic_sweep(which="all", copies=3, reps=6):
     the column-sliced form at the fixed tile height 4, so that every counter
     pass profiles the same kernel), and a 256 MiB device copy of known size
     that scales the counters. which="spmm" runs only the multi-RHS products
-    of the 192^3 variable-coefficient operator at k = 8, 2 and 1."""
+    of the 192^3 variable-coefficient operator at k = 8, 2 and 1 (FP64);
+    which="spmm-all" at k = 16, 8, 4, 2, 1 in FP64 and FP32."""
     import torch
 
     from diploma_research_amd import Csr5, matgen
 
-    if which == "spmm":
+    if which in ("spmm", "spmm-all"):
         # the multi-RHS product of the 192^3 variable-coefficient operator
         # (column-major blocks, tile height 7, column codes) at k = 8, 2, 1
+        # in FP64 (spmm-all: k = 16, 8, 4, 2, 1 in FP64 and FP32)
         side = 192
         n = side ** 3
         rp, ci, v = matgen.varcoef((side, side, side), 1.0, seed=4)
-        A = Csr5.from_csr((n, n, rp, ci, v), sigma=7)
+        A = Csr5.from_csr((n, n, rp, ci, v), sigma=7, keep_f32=which == "spmm-all")
         del rp, ci, v
-        for k in (8, 2, 1):
-            X = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, (k, n))).cuda().t()
-            Y = torch.empty((k, n), dtype=torch.float64, device="cuda").t()
-            for _ in range(reps + 2):
-                if k > 1:
-                    A.spmm(X, Y)
-                else:
-                    A.spmv(X[:, 0].contiguous(), Y[:, 0])
-            torch.cuda.synchronize()
-            log(f"spmm k={k} done")
-            del X, Y
+        ks = (16, 8, 4, 2, 1) if which == "spmm-all" else (8, 2, 1)
+        dts = (torch.float64, torch.float32) if which == "spmm-all" else (torch.float64,)
+        for dt in dts:
+            for k in ks:
+                X = torch.from_numpy(np.random.default_rng(1).uniform(-1, 1, (k, n))).to("cuda", dt).t()
+                Y = torch.empty((k, n), dtype=dt, device="cuda").t()
+                for _ in range(reps + 2):
+                    if k > 1:
+                        A.spmm(X, Y)
+                    else:
+                        A.spmv(X[:, 0].contiguous(), Y[:, 0])
+                torch.cuda.synchronize()
+                log(f"spmm {dt} k={k} done")
+                del X, Y
         return
     if which == "all":
         n = 128 ** 3
@@ -563,10 +568,11 @@ def main():
                     help="skip the oracle's CG solve of the 192^3 system (about a minute of CPU time)")
     ap.add_argument("--fd-entries", type=int, default=64,
                     help="stored entries the batched adjoint is checked on by finite differences (0: none)")
-    ap.add_argument("--traffic-sweep", nargs="?", const="all", choices=("all", "powerlaw", "spmm"),
+    ap.add_argument("--traffic-sweep", nargs="?", const="all", choices=("all", "powerlaw", "spmm", "spmm-all"),
                     help="run the products the rocprofv3 counter runs profile, and nothing else "
                          "(powerlaw: only the power-law product, column-sliced at tile height 4; "
-                         "spmm: the 192^3 multi-RHS products at k = 8, 2, 1)")
+                         "spmm: the 192^3 multi-RHS products at k = 8, 2, 1 in FP64; spmm-all: k = 16..1, "
+                         "FP64 and FP32)")
     ap.add_argument("--cpu-leg", metavar="DIR", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.cpu_leg:

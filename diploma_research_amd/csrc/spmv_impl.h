@@ -1210,8 +1210,10 @@ Plan make_plan(uint32_t ntiles) {
     if (!ntiles)
         return pl;
     const uint64_t waves = static_cast<uint64_t>(device_cus()) * blocks_per_cu * 4;
-    if (KMAX > 1 && sweep_tiles()) {
+    if (KMAX > 2 && sweep_tiles()) {
         // ordered sweep: short chunks, every resident wavefront looping
+        // (from 3 right-hand sides: at 2 the gathers of one chunk per
+        // wavefront already stay in L2)
         pl.sweep = true;
         pl.tpw = sweep_tiles();
         pl.nchunks = (ntiles + pl.tpw - 1) / pl.tpw;
