@@ -319,6 +319,19 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
     // right-hand side inside the tile, after the decode.
     constexpr bool XREG = KMAX == 1;
     constexpr bool REG = XREG || SIGMA <= kSmmRegSigma;
+    // Plain single-vector products stage the row sums of a clean tile's
+    // general path in LDS (one region per wavefront of the 256-thread block)
+    // and write the tile's rows with one coalesced pass: written where they
+    // are closed, the sums of short rows reach y as scattered 8-byte stores
+    // that fill each line of y piecemeal. Measured on the MI355X, power-law
+    // product (column-sliced, 2.5 elements per stacked row): those stores
+    // took 135 of its 596 us.
+    constexpr bool STAGE = KMAX == 1 && !DOT && SIGMA <= 8;
+    V* ystage = nullptr;
+    if constexpr (STAGE) {
+        __shared__ V ybuf[4][64 * SIGMA];
+        ystage = ybuf[(threadIdx.x >> 6) & 3];
+    }
     constexpr bool PIPE = XREG && SIGMA <= 8;
     constexpr bool CODED = CODES && REG && SIGMA <= 8;
     constexpr int NREG = REG ? SIGMA : 1;
@@ -705,6 +718,8 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                     if (s == 0 && !starts_row) {
                         own0 = true;
                         seg0 = val;
+                    } else if constexpr (STAGE && !DIRTY) {
+                        ystage[s] = val.a[0];
                     } else {
                         write_row(row_of(s), val);
                     }
@@ -844,6 +859,22 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                         for (int i = 0; i < GG; ++i)
                             carry[g + i] = tv.a[i];
                     }
+                }
+                if constexpr (STAGE && !DIRTY) {
+                    // Every segment in [s_lo, s_hi] was emitted exactly once
+                    // (segment 0 of a tile that continues a row went to the
+                    // carry or the head slot instead; the last segment stays
+                    // open unless the tile completes it): write them in order.
+                    // The wave barriers keep the compiler from moving these
+                    // LDS accesses across the other lanes' stores and the next
+                    // tile's; a wavefront's LDS instructions run in order.
+                    __builtin_amdgcn_wave_barrier();
+                    const int s_lo = starts_row ? 0 : 1;
+                    const int s_hi = complete ? nseg - 1 : nseg - 2;
+                    V* yrow = y + static_cast<size_t>(row0 - p.row_base);
+                    for (int i = s_lo + lane; i <= s_hi; i += 64)
+                        yrow[i] = ystage[i];
+                    __builtin_amdgcn_wave_barrier();
                 }
             }
             // segment 0 closed the carried row, or the whole tile did
