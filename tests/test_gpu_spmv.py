@@ -283,6 +283,36 @@ def test_poisson3d_128_full_size_closed_form(cuda):
     assert_spmv_close(A.spmv(to_dev(x)).cpu().numpy(), O.csr_spmv(rp, ci, v, x), rp, ci, v, x)
 
 
+@pytest.mark.parametrize("sigma", [4, 5, 6, 7, 8])
+@pytest.mark.parametrize("empty_frac", [0.0, 0.05])
+def test_short_rows_staged_writes_exact(cuda, sigma, empty_frac):
+    """Rows of 1-3 elements put most tiles on the general path, whose row sums
+    plain products of tile height <= 8 stage in LDS and write per tile in one
+    coalesced pass (tiles with empty rows, here 5% of the rows in one case,
+    keep their direct stores): with
+    integer values every summation order is exact, so y must equal the
+    oracle's bit for bit, in FP64 and FP32, including rows that cross tiles
+    and chunks, the padded last tile and a sliced form of the same matrix."""
+    import torch
+
+    rows, cols = 50_000, 40_000
+    rng = np.random.default_rng(sigma)
+    lens = rng.integers(1, 4, rows)  # 1-3 elements: clean tiles, most on the general path
+    lens[rng.random(rows) < empty_frac] = 0
+    lens[17], lens[rows - 1] = 900, 700  # rows across several tiles and chunks
+    rp = np.zeros(rows + 1, np.uint32)
+    np.cumsum(lens, out=rp[1:])
+    ci = np.concatenate([np.sort(rng.choice(cols, int(L), replace=False)) for L in lens]).astype(np.uint32)
+    v = rng.integers(-8, 9, len(ci)).astype(np.float64)
+    x = rng.integers(-4, 5, cols).astype(np.float64)
+    ref = O.csr_spmv(rp, ci, v, x)
+    A = Csr5.from_csr((rows, cols, rp, ci, v), sigma=sigma, keep_f32=True)
+    np.testing.assert_array_equal(A.spmv(to_dev(x)).cpu().numpy(), ref)
+    np.testing.assert_array_equal(A.spmv(to_dev(x, torch.float32)).double().cpu().numpy(), ref)
+    S = Csr5.from_csr((rows, cols, rp, ci, v), sigma=sigma, xcd_slices=True)
+    np.testing.assert_array_equal(S.spmv(to_dev(x)).cpu().numpy(), ref)
+
+
 def test_powerlaw_4m_full_size(cuda):
     """The irregular power-law matrix at n = 4,000,000 against the oracle."""
     n = 4_000_000
