@@ -416,14 +416,14 @@ __global__ __launch_bounds__(kThreads) void k_b_resid(Tables tb, const double* b
 constexpr int kPThreads = 1024;          // threads per block of the persistent kernel (16 wavefronts)
 constexpr int kPWaves = kPThreads / 64;
 // block_range_sum's order depends on the block size: the persistent kernel
-// sums p.q like k_b_update_r only with blocks of the same size
+// sums p.q like k_b_update_r only with blocks of the same size (its work
+// items are the launch sequence's blocks of kRowsPerBlock rows)
 static_assert(kPThreads == kThreads, "persistent and launch-sequence blocks differ");
-constexpr uint32_t kPRowsPerItem = 4096; // rows of one vector work item
 constexpr uint32_t kFlagStride = 32;     // barrier flags 128 bytes apart
 
 struct PersistParams {
     TileParams tp;              // q = A p with p.q per chunk over the chunk table
-    Tables tb;                  // vector work items (kPRowsPerItem rows of one system)
+    Tables tb;                  // vector work items: the launch sequence's blocks
     SysArrays sa;
     const uint32_t* sys_chunk;  // chunks of system s: [sys_chunk[s], sys_chunk[s + 1])
     double *x, *r, *p, *q;
@@ -773,7 +773,7 @@ struct DevMem {
 };
 
 // The chunk table of a batch: about one chunk per resident wavefront of the
-// persistent kernel, never two systems in one chunk. Sizes are balanced
+// product with its dot epilogue, never two systems in one chunk. Sizes are balanced
 // within each system; at most `waves` chunks in all when waves exceeds the
 // number of systems.
 void build_chunk_table(const std::vector<uint32_t>& sys_tile, uint64_t waves, std::vector<uint32_t>& chunk_tile,
@@ -920,9 +920,9 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
     const int occ = persistent ? persistent_occupancy(A->f.sigma, codes) : 0;
     if (persistent && occ < 1)
         return fail(DIM_UNSUPPORTED, "batched CG: the persistent kernel cannot be resident for this format");
-    // block tables: one or more blocks per system, none shared (the
-    // multi-kernel form's vector kernels and the residuals; the persistent
-    // kernel has its own, smaller items)
+    // block tables: one or more blocks of kRowsPerBlock rows per system, none
+    // shared (the vector kernels, the residuals and the persistent kernel's
+    // work items)
     std::vector<uint32_t> blk_sys, blk_r0, blk_r1, sys_blk;
     make_items(A, kRowsPerBlock, blk_sys, blk_r0, blk_r1, sys_blk);
     const uint32_t B = static_cast<uint32_t>(blk_sys.size());
@@ -987,23 +987,12 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
     DIM_HIP_TRY(hipGetLastError());
 
     if (persistent) {
-        // one cooperative launch runs every iteration
-        std::vector<uint32_t> pbs, pr0, pr1, psb;
-        make_items(A, kPRowsPerItem, pbs, pr0, pr1, psb);
-        Tables ptb{};
-        uint32_t *q_bs, *q_r0, *q_r1, *q_sb;
-        DIM_HIP_TRY(m.upload(q_bs, pbs));
-        DIM_HIP_TRY(m.upload(q_r0, pr0));
-        DIM_HIP_TRY(m.upload(q_r1, pr1));
-        DIM_HIP_TRY(m.upload(q_sb, psb));
-        ptb.blk_sys = q_bs;
-        ptb.blk_r0 = q_r0;
-        ptb.blk_r1 = q_r1;
-        ptb.sys_blk = q_sb;
+        // one cooperative launch runs every iteration; its vector work items
+        // are the launch sequence's blocks (same rows, same reduction order)
         double *rr_part = nullptr, *dot_chunk = nullptr;
         unsigned* bar = nullptr;
         int* abort = nullptr;
-        DIM_HIP_TRY(m.alloc(rr_part, pbs.size()));
+        DIM_HIP_TRY(m.alloc(rr_part, B));
         DIM_HIP_TRY(m.alloc(dot_chunk, A->f.n_chunk_tab));
         // one block per CU when the kernel fits there (every block resident)
         const uint32_t grid = static_cast<uint32_t>(detail::device_cus());
@@ -1018,7 +1007,7 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
         PersistParams P{};
         P.tp = detail::make_product_params(A->f, A->s, p, q);
         P.tp.dot_chunk = dot_chunk;
-        P.tb = ptb;
+        P.tb = tb;
         P.sa = sa;
         P.sys_chunk = A->d_sys_chunk;
         P.x = x;
@@ -1029,7 +1018,7 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
         P.dot_chunk = dot_chunk;
         P.fixup = A->f.chunk_tab_fixup ? 1 : 0;
         P.nchunks = A->f.n_chunk_tab;
-        P.n_items = static_cast<uint32_t>(pbs.size());
+        P.n_items = B;
         P.n_sys = S;
         P.max_iters = static_cast<int>(o.max_iters);
         P.st = st;
