@@ -42,10 +42,10 @@ def _ptr(t):
 
 
 def _dtype_code(t):
-    torch = _torch()
-    if t.dtype == torch.float64:
+    name = t.dtype.name if isinstance(t, np.ndarray) else str(t.dtype).replace("torch.", "")
+    if name == "float64":
         return N.DIM_F64
-    if t.dtype == torch.float32:
+    if name == "float32":
         return N.DIM_F32
     raise TypeError(f"unsupported dtype {t.dtype}")
 
@@ -73,6 +73,21 @@ def _check_tensor(t, name, numel, dtypes=("float64",), len_msg=None):
     if t.numel() != numel:
         raise ValueError(len_msg or f"{name} must have {numel} entries, got {t.numel()}")
     return t
+
+
+def _check_host(a, name, numel, dtypes=("float64",), len_msg=None):
+    """Validates a host (numpy) vector for the host-pointer entry points: an
+    ndarray of one of `dtypes`, C-contiguous, one-dimensional, with `numel`
+    entries. Raises TypeError or ValueError as _check_tensor does."""
+    if not isinstance(a, np.ndarray):
+        raise TypeError(f"{name} must be a numpy array, got {type(a).__name__}")
+    if a.dtype.name not in dtypes:
+        raise TypeError(f"{name} must have dtype {' or '.join(dtypes)}, got {a.dtype}")
+    if a.ndim != 1 or not a.flags.c_contiguous:
+        raise ValueError(f"{name} must be a contiguous vector")
+    if a.size != numel:
+        raise ValueError(len_msg or f"{name} must have {numel} entries, got {a.size}")
+    return a
 
 
 @dataclass
@@ -296,12 +311,16 @@ class Csr5:
     # ----------------------------------------------------------- products --
     def spmv(self, data, y=None, calibrator=None, stream=None):
         """csr5::spmv: y = A*x. Accepts SpmvData(x, y, calibrator) or (x, y);
-        with y omitted a new tensor is returned."""
-        torch = _torch()
+        with y omitted a new tensor is returned. x and y are GPU tensors, or
+        numpy arrays in host memory (dim_csr5_spmv_host: copied to the device
+        and back, y returned once written)."""
         if isinstance(data, SpmvData):
             x, y = data.x, data.y
         else:
             x = data
+        if isinstance(x, np.ndarray):
+            return self._spmv_host(x, y, stream)
+        torch = _torch()
         _check_tensor(x, "x", self.dimensions.cols, ("float64", "float32"), _X_COLS)
         if y is None:
             y = torch.empty(self.dimensions.rows, dtype=x.dtype, device=x.device)
@@ -310,6 +329,17 @@ class Csr5:
         _check_tensor(y, "y", self.dimensions.rows, ("float64", "float32"))
         N.check(N.lib().dim_csr5_spmv(self._h, _dtype_code(x), _ptr(x), x.numel(), _ptr(y), y.numel(),
                                       _stream_ptr(stream)))
+        return y
+
+    def _spmv_host(self, x, y, stream):
+        _check_host(x, "x", self.dimensions.cols, ("float64", "float32"), _X_COLS)
+        if y is None:
+            y = np.empty(self.dimensions.rows, dtype=x.dtype)
+        if x.dtype != getattr(y, "dtype", None):
+            raise TypeError("x and y must have the same dtype")
+        _check_host(y, "y", self.dimensions.rows, ("float64", "float32"))
+        N.check(N.lib().dim_csr5_spmv_host(self._h, _dtype_code(x), x.ctypes.data, x.size, y.ctypes.data, y.size,
+                                           _stream_ptr(stream)))
         return y
 
     def spmv_timed(self, x, y, stream=None):
@@ -424,11 +454,23 @@ def conjugate_gradient(A: Csr5, b, x=None, rtol=1e-8, max_iters=5000, precision=
     """conjugate_gradient(A, b, x) (thesis.tex:2133-2160): solves A x = b.
 
     b, x: float64 GPU tensors of length rows (x holds x0 and receives the
-    solution; zeros when omitted). precision: "f64" or "f32-refined" (FP32
-    iteration inside FP64 residual replacement; needs keep_f32). Returns
-    (x, CgResult); a breakdown (non-SPD matrix) raises DimError."""
-    torch = _torch()
+    solution; zeros when omitted), or numpy arrays in host memory (dim_cg_host:
+    the vectors are copied to the device and the solution back). precision:
+    "f64" or "f32-refined" (FP32 iteration inside FP64 residual replacement;
+    needs keep_f32). Returns (x, CgResult); a breakdown (non-SPD matrix)
+    raises DimError."""
     n = A.dimensions.rows
+    if isinstance(b, np.ndarray):
+        _check_host(b, "b", n)
+        x = np.zeros(n) if x is None else x
+        _check_host(x, "x", n)
+        hist = np.zeros(max_iters + 1) if history else None
+        opts = _cg_opts(rtol, max_iters, precision, check_every, refine_delta, hist)
+        res = N.CgResult()
+        N.check(N.lib().dim_cg_host(A._h, b.ctypes.data, x.ctypes.data, C.byref(opts), C.byref(res),
+                                    _stream_ptr(stream)), allow=(N.DIM_NOT_CONVERGED,))
+        return x, _result(res, hist)
+    torch = _torch()
     _check_tensor(b, "b", n)
     if x is None:
         x = torch.zeros_like(b)

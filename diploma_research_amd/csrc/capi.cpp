@@ -431,6 +431,18 @@ dim_status autotune_sigma(uint32_t rows, uint32_t cols, uint64_t nnz, const uint
 }
 
 } // namespace
+
+dim_status stage_host_vectors(dim_csr5* A, char** x, char** y) {
+    if (!A->host_stage) {
+        double* buf = nullptr;
+        DIM_HIP_TRY(dev_alloc(A, buf, size_t{A->f.cols} + A->f.rows));
+        A->host_stage = buf;
+    }
+    *x = static_cast<char*>(A->host_stage);
+    *y = *x + size_t{A->f.cols} * sizeof(double);
+    return DIM_SUCCESS;
+}
+
 } // namespace dimgpu
 
 using namespace dimgpu;
@@ -575,6 +587,31 @@ dim_status dim_csr5_spmv(const dim_csr5* A, dim_dtype dtype, const void* x, uint
         return fail(DIM_INVALID_ARGUMENT, "null vector");
     auto* self = const_cast<dim_csr5*>(A);
     DIM_HIP_TRY(plain_product(self, dtype == DIM_F32, x, y, static_cast<hipStream_t>(stream)));
+    return DIM_SUCCESS;
+}
+
+dim_status dim_csr5_spmv_host(const dim_csr5* A, dim_dtype dtype, const void* x, uint64_t x_len, void* y,
+                              uint64_t y_len, void* stream) {
+    if (dim_status st = check_handle(A))
+        return st;
+    if (dim_status st = check_dtype(A, dtype))
+        return st;
+    if (x_len != A->f.cols)
+        return fail(DIM_INVALID_ARGUMENT, "can't multiply, vector size doesn't match column count of the matrix");
+    if (y_len != A->f.rows)
+        return fail(DIM_INVALID_ARGUMENT, "output vector size doesn't match row count of the matrix");
+    if ((A->f.cols && !x) || (A->f.rows && !y))
+        return fail(DIM_INVALID_ARGUMENT, "null vector");
+    auto* self = const_cast<dim_csr5*>(A);
+    const auto st = static_cast<hipStream_t>(stream);
+    const size_t esz = dtype == DIM_F32 ? sizeof(float) : sizeof(double);
+    char *dx = nullptr, *dy = nullptr;
+    if (dim_status s = stage_host_vectors(self, &dx, &dy))
+        return s;
+    DIM_HIP_TRY(hipMemcpyAsync(dx, x, size_t{A->f.cols} * esz, hipMemcpyHostToDevice, st));
+    DIM_HIP_TRY(plain_product(self, dtype == DIM_F32, dx, dy, st));
+    DIM_HIP_TRY(hipMemcpyAsync(y, dy, size_t{A->f.rows} * esz, hipMemcpyDeviceToHost, st));
+    DIM_HIP_TRY(hipStreamSynchronize(st));
     return DIM_SUCCESS;
 }
 

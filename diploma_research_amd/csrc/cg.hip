@@ -795,6 +795,26 @@ dim_status dim_cg(const dim_csr5* A, const double* b, double* x, const dim_cg_op
     return run(const_cast<dim_csr5*>(A), 1, b, x, opts, res, static_cast<hipStream_t>(stream));
 }
 
+dim_status dim_cg_host(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts, dim_cg_result* res,
+                       void* stream) {
+    if (dim_status st = check_cg_args(A, 1, b, x, opts, res))
+        return st;
+    auto* self = const_cast<dim_csr5*>(A);
+    const auto s = static_cast<hipStream_t>(stream);
+    char *db = nullptr, *dx = nullptr;
+    if (dim_status st = stage_host_vectors(self, &db, &dx))
+        return st;
+    const size_t bytes = size_t{A->f.rows} * sizeof(double);
+    DIM_HIP_TRY(hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, s));
+    DIM_HIP_TRY(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, s));
+    const dim_status st = run(self, 1, reinterpret_cast<const double*>(db), reinterpret_cast<double*>(dx), opts, res, s);
+    if (st != DIM_SUCCESS && st != DIM_NOT_CONVERGED && st != DIM_BREAKDOWN)
+        return st;
+    DIM_HIP_TRY(hipMemcpyAsync(x, dx, bytes, hipMemcpyDeviceToHost, s));
+    DIM_HIP_TRY(hipStreamSynchronize(s));
+    return st;
+}
+
 dim_status dim_cg_noalloc(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts,
                           dim_cg_result* res, void* stream) {
     if (dim_status st = check_cg_args(A, 1, b, x, opts, res))

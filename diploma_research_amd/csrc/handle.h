@@ -43,6 +43,10 @@ struct dim_csr5 {
     // grown on demand
     void* mm_scratch = nullptr;
     size_t mm_bytes = 0;
+    // device copies of host vectors for the host-pointer entry points
+    // (dim_csr5_spmv_host, dim_cg_host): cols + rows FP64-sized entries,
+    // allocated on first use
+    void* host_stage = nullptr;
 };
 
 namespace dimgpu {
@@ -50,6 +54,11 @@ namespace dimgpu {
 void set_error(const std::string& msg);
 dim_status fail(dim_status st, const std::string& msg);
 dim_status hip_fail(hipError_t e, const char* what);
+
+// The staging area of the host-pointer entry points: cols + rows entries of
+// 8 bytes on the matrix's device (*x at the start, *y after it), allocated
+// with the handle's arrays on first use.
+dim_status stage_host_vectors(dim_csr5* A, char** x, char** y);
 
 #define DIM_HIP_TRY(expr)                                                                                          \
     do {                                                                                                           \

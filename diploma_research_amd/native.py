@@ -35,8 +35,8 @@ DIM_CG_NOALLOC_MAX_ROWS = 2097152
 EXPORTS = [
     "dim_last_error", "dim_status_string", "dim_version",
     "dim_csr5_from_csr", "dim_csr5_from_csr_device", "dim_csr5_destroy", "dim_csr5_get_info",
-    "dim_csr5_copy_format", "dim_csr5_spmv", "dim_csr5_spmv_timed", "dim_csr5_block_rows", "dim_csr5_spmv_block", "dim_csr5_spmm",
-    "dim_csr5_spmm_colmajor", "dim_csr5_spmv_transpose", "dim_csr5_sampled_outer", "dim_cg", "dim_cg_noalloc",
+    "dim_csr5_copy_format", "dim_csr5_spmv", "dim_csr5_spmv_host", "dim_csr5_spmv_timed", "dim_csr5_block_rows", "dim_csr5_spmv_block", "dim_csr5_spmm",
+    "dim_csr5_spmm_colmajor", "dim_csr5_spmv_transpose", "dim_csr5_sampled_outer", "dim_cg", "dim_cg_host", "dim_cg_noalloc",
     "dim_cg_multi", "dim_csr5_from_csr_batched",
     "dim_cg_batched", "dim_csr5_n_systems", "dim_csr5_set_values", "dim_stream_read",
 ]
@@ -94,6 +94,7 @@ def lib():
         "dim_csr5_get_info": ([vp, P(Csr5Info)], i32),
         "dim_csr5_copy_format": ([vp, vp, vp, vp, vp, vp, P(u64), vp], i32),
         "dim_csr5_spmv": ([vp, i32, vp, u64, vp, u64, vp], i32),
+        "dim_csr5_spmv_host": ([vp, i32, vp, u64, vp, u64, vp], i32),
         "dim_csr5_spmv_timed": ([vp, i32, vp, u64, vp, u64, vp, P(C.c_float)], i32),
         "dim_csr5_block_rows": ([vp, u32, u32, P(u32), P(u32)], i32),
         "dim_csr5_spmv_block": ([vp, i32, u32, u32, vp, u64, vp, u64, vp], i32),
@@ -102,6 +103,7 @@ def lib():
         "dim_csr5_spmv_transpose": ([vp, i32, vp, vp, vp], i32),
         "dim_csr5_sampled_outer": ([vp, f64, vp, vp, vp, vp], i32),
         "dim_cg": ([vp, vp, vp, P(CgOpts), P(CgResult), vp], i32),
+        "dim_cg_host": ([vp, vp, vp, P(CgOpts), P(CgResult), vp], i32),
         "dim_cg_noalloc": ([vp, vp, vp, P(CgOpts), P(CgResult), vp], i32),
         "dim_cg_multi": ([vp, u32, vp, vp, P(CgOpts), P(CgResult), vp], i32),
         "dim_csr5_from_csr_batched": ([u32, vp, vp, vp, vp, vp, P(Csr5Opts), P(vp)], i32),

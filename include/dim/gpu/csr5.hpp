@@ -6,7 +6,8 @@
 //   reference                                   here
 //   csr5<double>::from_csr(csr)                 csr5::from_csr(csr) (host CSR in, device format out)
 //   csr5::create_calibrator()                   csr5::create_calibrator() (a token: carries live on the device)
-//   csr5::spmv<absolute>({x, y, calibrator})    csr5::spmv({x, y, calibrator}) with DEVICE spans
+//   csr5::spmv<absolute>({x, y, calibrator})    csr5::spmv({x, y, calibrator}) with DEVICE spans, or
+//                                               csr5::spmv_host({x, y, calibrator}) with host spans
 //   csr5::spmv<partial>(...) on a part from     the same, on a part from dim::io::h5::load_csr5_partial
 //     h5::load_csr5_partial                       (dim/gpu/h5.hpp), y holding last_row_idx() -
 //                                                 first_row_idx() + 1 entries; or csr5::spmv_block(
@@ -143,6 +144,16 @@ class csr5 {
         check(dim_csr5_spmv(h_.get(), DIM_F32, d.x.data(), d.x.size(), d.y.data(), d.y.size(), stream));
     }
 
+    // The same product with x and y in HOST memory, as the reference's
+    // callers hold them (cache_aligned_vector, benchmarks/csr5/dim/main.cpp:
+    // 58-68): copied to the device and back, returns once y is written.
+    void spmv_host(::dim::gpu::spmv_data_t<double> d, void* stream = nullptr) const {
+        check(dim_csr5_spmv_host(h_.get(), DIM_F64, d.x.data(), d.x.size(), d.y.data(), d.y.size(), stream));
+    }
+    void spmv_host(::dim::gpu::spmv_data_t<float> d, void* stream = nullptr) const {
+        check(dim_csr5_spmv_host(h_.get(), DIM_F32, d.x.data(), d.x.size(), d.y.data(), d.y.size(), stream));
+    }
+
     // spmv<partial> on tiles [tile_begin, tile_end): y_block has
     // block_rows(...) entries starting at row first_row_idx
     struct block_t {
@@ -221,6 +232,20 @@ inline dim_cg_result conjugate_gradient(const csr5& A, const double* b, double* 
     opts.precision = o.f32_refined ? DIM_CG_F32_REFINED : DIM_CG_F64;
     dim_cg_result res{};
     const dim_status st = dim_cg(A.handle(), b, x, &opts, &res, stream);
+    if (st != DIM_NOT_CONVERGED)
+        check(st);
+    return res;
+}
+
+// The same solve with b and x in HOST memory (dim_cg_host).
+inline dim_cg_result conjugate_gradient_host(const csr5& A, const double* b, double* x, const cg_options& o = {},
+                                             void* stream = nullptr) {
+    dim_cg_opts opts{};
+    opts.rtol = o.rtol;
+    opts.max_iters = o.max_iters;
+    opts.precision = o.f32_refined ? DIM_CG_F32_REFINED : DIM_CG_F64;
+    dim_cg_result res{};
+    const dim_status st = dim_cg_host(A.handle(), b, x, &opts, &res, stream);
     if (st != DIM_NOT_CONVERGED)
         check(st);
     return res;

@@ -112,6 +112,15 @@ dim_status dim_csr5_copy_format(const dim_csr5* A, uint32_t* tile_ptr, uint32_t*
 dim_status dim_csr5_spmv(const dim_csr5* A, dim_dtype dtype, const void* x, uint64_t x_len, void* y,
                          uint64_t y_len, void* stream);
 
+/* csr5::spmv (csr5.h:818-829) for callers that hold x and y in HOST memory,
+ * as the reference's callers do (benchmarks/csr5/dim/main.cpp:58-68): copies
+ * x to the device, runs dim_csr5_spmv's kernels and copies y back; returns
+ * when y is written. The device copies live in the handle (cols + rows
+ * entries of 8 bytes, allocated on the first call); the same argument rules
+ * as dim_csr5_spmv. */
+dim_status dim_csr5_spmv_host(const dim_csr5* A, dim_dtype dtype, const void* x, uint64_t x_len, void* y,
+                              uint64_t y_len, void* stream);
+
 /* Measurement variant of dim_csr5_spmv: records HIP events around the tile
  * kernel and the carry/fix-up kernel on `stream`, waits for them and returns
  * both durations in milliseconds (ms[0] tiles, ms[1] fix-up; ms[1] is 0 when
@@ -198,6 +207,13 @@ typedef struct dim_cg_result {
  * entry. Synchronises `stream` before returning. */
 dim_status dim_cg(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts, dim_cg_result* res,
                   void* stream);
+
+/* dim_cg with b and x in HOST memory (benchmarks/conjugate_gradient/dim_sp/
+ * main.cpp:84-127 keeps its vectors on the host): x holds x0 on entry and
+ * the solution on return (also when the solve stops unconverged). Staged
+ * through the handle's device copies, as dim_csr5_spmv_host. */
+dim_status dim_cg_host(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts, dim_cg_result* res,
+                       void* stream);
 
 /* The solve of dim_cg (one right-hand side, DIM_CG_F64 or DIM_CG_F32_REFINED)
  * for callers that cannot allocate device memory: loops that re-solve many

@@ -71,6 +71,30 @@ def test_cg_coded_products_of_short_chunks(cuda, n):
     assert res.relres_true <= 2e-8
 
 
+def test_cg_host_pointers_equal_device_solve(cuda):
+    """dim_cg_host (b and x in host memory, x0 nonzero, with history) returns
+    the device solve's iterate, result and history bit for bit; a solve that
+    stops at max_iters still hands back its iterate."""
+    n = 48
+    rp, ci, v = matgen.poisson2d(n)
+    N = n * n
+    A = Csr5.from_csr((N, N, rp, ci, v), sigma=0)
+    rng = np.random.default_rng(21)
+    b = rng.uniform(-1, 1, N)
+    x0 = rng.uniform(-0.1, 0.1, N)
+    xd, rd = conjugate_gradient(A, to_dev(b), to_dev(x0), rtol=1e-10, history=True)
+    xh, rh = conjugate_gradient(A, b, x0.copy(), rtol=1e-10, history=True)
+    assert isinstance(xh, np.ndarray)
+    np.testing.assert_array_equal(xh, xd.cpu().numpy())
+    assert (rh.iterations, rh.converged, rh.relres, rh.relres_true) == \
+        (rd.iterations, rd.converged, rd.relres, rd.relres_true)
+    np.testing.assert_array_equal(rh.history, rd.history)
+    xc, rc = conjugate_gradient(A, b, None, rtol=1e-14, max_iters=7)
+    xcd, _ = conjugate_gradient(A, to_dev(b), None, rtol=1e-14, max_iters=7)
+    assert not rc.converged and rc.iterations == 7
+    np.testing.assert_array_equal(xc, xcd.cpu().numpy())
+
+
 def test_cg_is_bitwise_reproducible(cuda):
     rp, ci, v = matgen.poisson3d(40)
     n = 40 ** 3

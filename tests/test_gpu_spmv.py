@@ -313,6 +313,34 @@ def test_short_rows_staged_writes_exact(cuda, sigma, empty_frac):
     np.testing.assert_array_equal(S.spmv(to_dev(x)).cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("sliced", [False, True])
+def test_host_pointer_product_equals_device_product(cuda, sliced):
+    """dim_csr5_spmv_host (numpy vectors in host memory, staged through the
+    handle) gives the device product's y bit for bit, FP64 and FP32, on
+    repeated calls, and rejects wrong sizes and dtypes like the device path."""
+    import torch
+
+    rp, ci, v = matgen.random_csr(5000, 4000, seed=3, max_len=12, empty_frac=0.1, long_rows=[(9, 3000)])
+    A = Csr5.from_csr((5000, 4000, rp, ci, v), sigma=8, keep_f32=True, xcd_slices=sliced)
+    rng = np.random.default_rng(4)
+    for _ in range(2):
+        x = rng.uniform(-1, 1, 4000)
+        y_host = A.spmv(x)
+        assert isinstance(y_host, np.ndarray)
+        np.testing.assert_array_equal(y_host, A.spmv(to_dev(x)).cpu().numpy())
+        x32 = x.astype(np.float32)
+        np.testing.assert_array_equal(A.spmv(x32), A.spmv(to_dev(x32)).cpu().numpy())
+    y = np.empty(5000)
+    assert A.spmv(x, y) is y
+    with pytest.raises(ValueError, match="column count"):
+        A.spmv(np.zeros(3999))
+    with pytest.raises(TypeError):
+        A.spmv(np.zeros(4000, dtype=np.int64))
+    with pytest.raises(TypeError):
+        A.spmv(x, np.empty(5000, dtype=np.float32))
+    assert torch.cuda.is_available()
+
+
 def test_powerlaw_4m_full_size(cuda):
     """The irregular power-law matrix at n = 4,000,000 against the oracle."""
     n = 4_000_000
