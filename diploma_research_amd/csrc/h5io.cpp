@@ -570,6 +570,8 @@ dim_status dim_h5_load_csr5_partial(const char* path, const char* group, uint32_
     p.tile_count = nt;
     p.val_offset = v0;
     p.skip_tail = last_part ? 0 : 1;
+    // the part's first element continues a row that began in an earlier part
+    p.first_tile_uncapped = !rp.empty() && rp[0] != v0 ? 1 : 0;
     p.block.rows = n_rows;
     p.block.cols = cols;
     p.block.nnz = ne;

@@ -41,7 +41,7 @@ class Csr5Props(C.Structure):
 class H5Csr5Part(C.Structure):
     _fields_ = [("rows", C.c_uint32), ("cols", C.c_uint32), ("first_row", C.c_uint32), ("last_row", C.c_uint32),
                 ("tile_begin", C.c_uint64), ("tile_count", C.c_uint64), ("val_offset", C.c_uint64),
-                ("skip_tail", C.c_int32), ("block", H5Csr)]
+                ("skip_tail", C.c_int32), ("block", H5Csr), ("first_tile_uncapped", C.c_int32)]
 
 
 class H5Error(RuntimeError):
@@ -123,7 +123,8 @@ def load_csr5_partial(path, group, part_idx, part_count) -> dict:
     """load_csr5_partial (libs/dim/io/h5.cpp:137-173): part part_idx of
     part_count of a CSR5 group's tiles, as one rank loads it. Returns the
     whole matrix's rows/cols, first_row / last_row (first_row_idx(),
-    last_row_idx()), tile_begin, tile_count, val_offset, skip_tail and
+    last_row_idx()), tile_begin, tile_count, val_offset, skip_tail,
+    first_tile_uncapped and
     `block` = (n_rows, cols, row_ptr, col_idx, vals): the part's elements as
     CSR over rows first_row .. last_row, whose product is the part's share
     of y (edge rows partial)."""
@@ -136,7 +137,7 @@ def load_csr5_partial(path, group, part_idx, part_count) -> dict:
         v = np.ctypeslib.as_array(b.vals, (b.nnz,)).copy() if b.nnz else np.zeros(0)
         return dict(rows=m.rows, cols=m.cols, first_row=m.first_row, last_row=m.last_row, tile_begin=m.tile_begin,
                     tile_count=m.tile_count, val_offset=m.val_offset, skip_tail=bool(m.skip_tail),
-                    block=(int(b.rows), int(b.cols), rp, ci, v))
+                    first_tile_uncapped=bool(m.first_tile_uncapped), block=(int(b.rows), int(b.cols), rp, ci, v))
     finally:
         lib().dim_h5_free_csr5_part(C.byref(m))
 

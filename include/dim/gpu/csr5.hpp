@@ -97,6 +97,7 @@ class csr5 {
         uint32_t first_row = 0, n_rows = 0;
         uint64_t val_offset = 0;
         bool skip_tail = false;
+        bool first_tile_uncapped = false;
     };
 
     // takes ownership of a handle from the C ABI; `part` marks a loaded share
@@ -194,6 +195,9 @@ class csr5 {
         return is_part_ ? part_.first_row + part_.n_rows - 1 : dimensions().rows - 1;
     }
     uint64_t val_offset() const { return is_part_ ? part_.val_offset : 0u; }
+    // csr5.h:845-847: the first element does not start a row (only a loaded
+    // part can begin inside a row)
+    bool first_tile_uncapped() const { return is_part_ && part_.first_tile_uncapped; }
     bool skip_tail() const { return is_part_ && part_.skip_tail; }
     uint32_t sigma() const { return info().sigma; }
     static constexpr uint32_t omega() { return 64; }

@@ -54,7 +54,8 @@ inline ::dim::gpu::csr5 load_csr5_partial(const std::filesystem::path& path, con
     dim_csr5* h = nullptr;
     const dim_status st = dim_csr5_from_csr(p.block.rows, p.block.cols, p.block.nnz, p.block.row_ptr,
                                             p.block.col_idx, p.block.vals, &opts, &h);
-    const ::dim::gpu::csr5::part_t info{p.first_row, p.block.rows, p.val_offset, p.skip_tail != 0};
+    const ::dim::gpu::csr5::part_t info{p.first_row, p.block.rows, p.val_offset, p.skip_tail != 0,
+                                        p.first_tile_uncapped != 0};
     dim_h5_free_csr5_part(&p);
     ::dim::gpu::check(st);
     return ::dim::gpu::csr5::adopt(h, &info);

@@ -105,6 +105,8 @@ typedef struct dim_h5_csr5_part {
     uint64_t val_offset;   /* index of its first element in the group's vals */
     int32_t skip_tail;     /* 1 unless this is the last part */
     dim_h5_csr block;
+    int32_t first_tile_uncapped; /* first_tile_uncapped() (csr5.h:845-847): the part's first element does
+                                    not start a row (the row began in an earlier part) */
 } dim_h5_csr5_part;
 dim_status dim_h5_load_csr5_partial(const char* path, const char* group, uint32_t part_idx, uint32_t part_count,
                                     dim_h5_csr5_part* out);

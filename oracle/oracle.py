@@ -415,11 +415,12 @@ def ref_h5_load_csr5_partial(path, group, idx, count, x=None, nthreads=4) -> dic
     if not h:
         raise RuntimeError("reference load_csr5_partial failed")
     try:
-        info = np.zeros(9, np.uint64)
+        info = np.zeros(10, np.uint64)
         L.dimref_h5_part_info(h, info)
-        rows, cols, first, last, val_offset, skip_tail, T, nv, nrp = (int(i) for i in info)
+        rows, cols, first, last, val_offset, skip_tail, T, nv, nrp, uncapped = (int(i) for i in info)
         out = dict(rows=rows, cols=cols, first_row=first, last_row=last, val_offset=val_offset,
-                   skip_tail=bool(skip_tail), tile_count=T, row_ptr=np.zeros(nrp, np.uint32),
+                   skip_tail=bool(skip_tail), first_tile_uncapped=bool(uncapped), tile_count=T,
+                   row_ptr=np.zeros(nrp, np.uint32),
                    tile_ptr=np.zeros(T + 1, np.uint32), vals=np.zeros(nv), col_idx=np.zeros(nv, np.uint32))
         L.dimref_h5_part_copy(h, out["row_ptr"], out["tile_ptr"], out["vals"], out["col_idx"])
         if x is not None:

@@ -114,18 +114,20 @@ void* dimref_h5_load_csr5_partial(const char* path, const char* group, uint64_t 
 }
 
 // rows, cols (global), first_row_idx(), last_row_idx() (csr5.h:845-855),
-// val_offset, skip_tail, tile_count, loaded elements, loaded row_ptr entries
-void dimref_h5_part_info(void* h, uint64_t* out9) {
+// val_offset, skip_tail, tile_count, loaded elements, loaded row_ptr entries,
+// first_tile_uncapped() (csr5.h:845-847)
+void dimref_h5_part_info(void* h, uint64_t* out10) {
     const auto& m = *static_cast<csr5_t*>(h);
-    out9[0] = m.dimensions.rows;
-    out9[1] = m.dimensions.cols;
-    out9[2] = m.first_row_idx();
-    out9[3] = m.last_row_idx();
-    out9[4] = m.val_offset;
-    out9[5] = m.skip_tail ? 1 : 0;
-    out9[6] = m.csr5_info.tile_count;
-    out9[7] = m.vals.size();
-    out9[8] = m.row_ptr.size();
+    out10[0] = m.dimensions.rows;
+    out10[1] = m.dimensions.cols;
+    out10[2] = m.first_row_idx();
+    out10[3] = m.last_row_idx();
+    out10[4] = m.val_offset;
+    out10[5] = m.skip_tail ? 1 : 0;
+    out10[6] = m.csr5_info.tile_count;
+    out10[7] = m.vals.size();
+    out10[8] = m.row_ptr.size();
+    out10[9] = m.first_tile_uncapped() ? 1 : 0;
 }
 
 // the loaded slabs: row_ptr (absolute element indices), tile_ptr (raw words),

@@ -43,7 +43,7 @@ def _valid(T, i, count):
 @pytest.mark.parametrize("name", ["dirty", "poisson", "sparse_rect", "fast_track"])
 def test_partial_load_matches_reference(tmp_path, name):
     """Every part of 1, 2, 3, 5, 8 and 13: the same first/last row,
-    val_offset, skip_tail and tile count as the reference's loader; the block
+    val_offset, skip_tail, first_tile_uncapped and tile count as the reference's loader; the block
     holds the part's elements in CSR order (the in-memory partition of the same
     CSR); the block's product equals the reference's spmv<partial> of the part
     within the parity gate; parts the reference cannot load (it wraps around
@@ -54,6 +54,7 @@ def test_partial_load_matches_reference(tmp_path, name):
     x = np.random.default_rng(1).uniform(-1, 1, cols)
     y_full = O.csr_spmv(rp, ci, v, x)
     T = len(v) // 64
+    uncapped_seen = set()
     for count in (1, 2, 3, 5, 8, 13):
         total = np.zeros(rows)
         complete = True
@@ -65,8 +66,10 @@ def test_partial_load_matches_reference(tmp_path, name):
                 continue
             ref = O.ref_h5_load_csr5_partial(path, "A", i, count, x=x)
             got = h5.load_csr5_partial(path, "A", i, count)
-            for key in ("rows", "cols", "first_row", "last_row", "val_offset", "skip_tail", "tile_count"):
+            for key in ("rows", "cols", "first_row", "last_row", "val_offset", "skip_tail", "tile_count",
+                        "first_tile_uncapped"):
                 assert got[key] == ref[key], (count, i, key, got[key], ref[key])
+            uncapped_seen.add(got["first_tile_uncapped"])
             first, v0, blk = h5.partition_block(rows, cols, rp, ci, v, i, count)
             assert (first, v0) == (got["first_row"], got["val_offset"])
             n, c, brp, bci, bv = got["block"]
@@ -82,6 +85,8 @@ def test_partial_load_matches_reference(tmp_path, name):
             total[first:first + n] += yb
         if complete:
             np.testing.assert_allclose(total, y_full, rtol=0, atol=1e-12 * max(np.abs(y_full).max(), 1.0))
+    if name == "poisson":  # short rows: parts begin both at a row start and inside a row
+        assert uncapped_seen == {False, True}
 
 
 @needs_ref
