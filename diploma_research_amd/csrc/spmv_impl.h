@@ -405,12 +405,12 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
             }
         }
     };
-    // Coded single-vector tiles stream 8.5 bytes per element instead of 12;
-    // up to sigma 7 their pipeline runs two tiles ahead (two stage slots, the
-    // chunk walked in pairs so that each slot keeps its registers). Measured
-    // on the 7-point operator: sigma 7 gains 4%, sigma 8 loses 5% to the
-    // lower occupancy.
-    constexpr bool PIPE2 = PIPE && CODED && SIGMA <= 7;
+    // Up to sigma 7 the single-vector pipeline runs two tiles ahead (two
+    // stage slots, the chunk walked in pairs so that each slot keeps its
+    // registers). Measured on the 7-point operator: coded sigma 7 gains 4%,
+    // coded sigma 8 loses 5% to the lower occupancy; 32-bit columns at
+    // sigma 7: 45.8 -> 45.0 us.
+    constexpr bool PIPE2 = PIPE && SIGMA <= 7;
     constexpr uint32_t AHEAD = PIPE2 ? 2 : 1;
     Stage nx, nx2;
     if constexpr (PIPE)
