@@ -174,6 +174,25 @@ def test_spmm_equals_columnwise_spmv(cuda, k, sigma):
     np.testing.assert_array_equal(Y32, Y)  # small integers are exact in FP32 too
 
 
+@pytest.mark.parametrize("rows", [1, 40, 300])
+def test_spmm_sweep_with_fewer_chunks_than_xcds(cuda, rows):
+    """Multi-RHS products (k > 2 take the ordered sweep) of matrices with one
+    to a few chunks, so that most XCD ranges are empty: repeated products
+    (and products of another matrix in between) must each be exact, which
+    they are only if every launch leaves the chunk counters at zero."""
+    rng = np.random.default_rng(rows)
+    rp, ci, v = matgen.random_csr(rows, 50, seed=rows, max_len=6, integer=True)
+    A = Csr5.from_csr((rows, 50, rp, ci, v), sigma=4)
+    B = Csr5.from_csr((rows, 50, rp, ci, v), sigma=8)
+    for rep in range(3):
+        for M in (A, B):
+            for k in (3, 8, 16):
+                X = rng.integers(-4, 5, (50, k)).astype(np.float64)
+                Y = M.spmm(to_dev(X)).cpu().numpy()
+                for j in range(k):
+                    np.testing.assert_array_equal(Y[:, j], O.csr_spmv(rp, ci, v, X[:, j].copy()), err_msg=f"{rep} {k}")
+
+
 @pytest.mark.parametrize("k", [2, 3, 8, 13, 16])
 @pytest.mark.parametrize("sigma", [4, 7, 8])
 @pytest.mark.parametrize("layout", ["row", "col", "col_ld"])
