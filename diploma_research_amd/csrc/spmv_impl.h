@@ -1019,20 +1019,22 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         const bool sweep = p.queue != nullptr;
         const uint32_t xr = blockIdx.x % kXcds;
         uint32_t* q = sweep ? p.queue + xr * kQueueStride : nullptr;
-        uint32_t c0 = 0, c1 = p.nchunks, c;
+        // chunk c0 + i for i < n (the index is compared before it is added,
+        // so no counter value can wrap into the range)
+        uint32_t c0 = 0, n = p.nchunks, i;
         if (sweep) {
             const uint32_t per = (p.nchunks + kXcds - 1) / kXcds;
             c0 = min(xr * per, p.nchunks);
-            c1 = min(c0 + per, p.nchunks);
-            c = c0 + __builtin_amdgcn_readfirstlane(atomicAdd(q, lane == 0 ? 1u : 0u));
+            n = min(c0 + per, p.nchunks) - c0;
+            i = __builtin_amdgcn_readfirstlane(atomicAdd(q, lane == 0 ? 1u : 0u));
         } else {
-            c = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+            i = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
         }
-        while (c < c1) { // wave-uniform
-            (void)csr5_chunk<V, SIGMA, KMAX, DOT, CODES>(p, c, false);
+        while (i < n) { // wave-uniform
+            (void)csr5_chunk<V, SIGMA, KMAX, DOT, CODES>(p, c0 + i, false);
             if (!sweep)
                 break;
-            c = c0 + __builtin_amdgcn_readfirstlane(atomicAdd(q, lane == 0 ? 1u : 0u));
+            i = __builtin_amdgcn_readfirstlane(atomicAdd(q, lane == 0 ? 1u : 0u));
         }
         if (sweep) {
             // wavefronts of this range: 4 per block b < gridDim.x with
