@@ -199,6 +199,19 @@ uint32_t detail::sweep_tiles() {
     return 4;
 }
 
+bool detail::solve_streams_matrix(const DevFormat& A, size_t vb) {
+    constexpr uint64_t kInfinityCacheBytes = 256ull << 20;
+    const uint64_t lanes = static_cast<uint64_t>(A.tiles) * 64;
+    const uint64_t elems = lanes * A.sigma;
+    uint64_t bytes = elems * vb; // values
+    if (A.col_code)
+        bytes += lanes * 4; // one code word per lane, flags included
+    else
+        bytes += elems * 4 + lanes * (A.sigma <= 16 ? 2 : 4); // column indices, row-start flags
+    bytes += 4ull * A.rows * vb;                               // x, r, p, q
+    return bytes > kInfinityCacheBytes;
+}
+
 int detail::device_cus() {
     static int cus[64] = {};
     int dev = 0;

@@ -142,9 +142,14 @@ __device__ __forceinline__ V wave_sum(V v) {
 // Matrix values and column indices are read once per product. A plain
 // product streams them with non-temporal loads, so that they do not evict the
 // x entries that neighbouring rows gather again from L2 (measured: 10% less
-// HBM read traffic on the 7-point operator). Products fused with CG's dot use
-// ordinary loads: the solver multiplies the same matrix every iteration, and
-// a working set that fits the 256 MB Infinity Cache then stays there.
+// HBM read traffic on the 7-point operator). Products fused with CG's dot
+// (NT = false) use ordinary loads when the solver's working set, the matrix
+// and four vectors, fits the 256 MB Infinity Cache: the solver multiplies the
+// same matrix every iteration, and it then stays there (128^3 Poisson CG:
+// 20.4 ms, 22.8 ms with non-temporal loads). Single-vector solves whose
+// working set does not fit stream it non-temporally like a plain product
+// (32 uneven 2-D systems, 427 MB: 0.586 s against 0.611 s); see
+// solve_streams_matrix().
 template<bool NT, typename V>
 __device__ __forceinline__ V ldg_stream(const V* p) {
     if constexpr (NT)
@@ -247,7 +252,7 @@ constexpr int kSmmRegSigma = 16;
 // them.
 // CODES: the matrix carries column codes (every tile clean, sigma <= 8); the
 // kernel then streams one code word per lane instead of sigma column indices.
-template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES = false>
+template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES = false, bool NT = !DOT>
 __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t c, const bool stopped) {
     const int lane = threadIdx.x & 63;
     uint32_t t0, t1;
@@ -363,12 +368,12 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
             const uint32_t* ct = p.col_idx + base;
 #pragma unroll
             for (int r = 0; r < SIGMA; ++r) {
-                s.v[r] = ldg_stream<!DOT>(vt + r * 64 + ulane);
+                s.v[r] = ldg_stream<NT>(vt + r * 64 + ulane);
                 if constexpr (!CODED)
-                    s.col[r] = ldg_stream<!DOT>(ct + r * 64 + ulane);
+                    s.col[r] = ldg_stream<NT>(ct + r * 64 + ulane);
             }
             if constexpr (CODED)
-                s.code = ldg_stream<!DOT>(p.col_code + tile64 + ulane);
+                s.code = ldg_stream<NT>(p.col_code + tile64 + ulane);
         }
         if constexpr (!CODED) // a code word carries its lane's flags
             s.bits = (bits_all + tile64)[ulane];
@@ -569,17 +574,19 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                 // The rows a lane may write: segment y_off - 1 (tot), y_off
                 // (mid; in lane 0 segment 0; in lane 63 with one start the
                 // tile's last segment) and y_off + 1 (the tile's last segment
-                // in lane 63 with two starts). The fused dot reads x at them.
-                const size_t ra = row0 + static_cast<uint32_t>(y_off > 0 ? y_off - 1 : 0);
+                // in lane 63 with two starts). The fused dot needs x at them:
+                // a lane loads x at its own starts (rows y_off and, with two
+                // starts, y_off + 1) and takes x at row y_off - 1, the last
+                // start of the lane below (every lane holds one or two
+                // starts here), from that lane.
                 const size_t rb = row0 + static_cast<uint32_t>(y_off);
-                const size_t rc = row0 + static_cast<uint32_t>(y_off + 1 < nseg ? y_off + 1 : y_off);
                 constexpr int G = KMAX < smm_group<V>() ? KMAX : smm_group<V>();
 #pragma unroll
                 for (int g = 0; g < KMAX; g += G) {
                     if (g < k) {
                         // sides past k re-read side k - 1 and store nothing
                         V xv[G][SIGMA];
-                        V px[G][3];
+                        V px[G][2]; // x at the lane's first and last row starts (the dot)
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
                             const V* xj = x + static_cast<size_t>(g + i < k ? g + i : k - 1) * xcs;
@@ -591,9 +598,10 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                                     xv[i][r] = xj[static_cast<size_t>(cu.col[r]) * rs];
                             }
                             if constexpr (DOT) {
-                                px[i][0] = xj[ra * rs];
-                                px[i][1] = xj[rb * rs];
-                                px[i][2] = xj[rc * rs];
+                                // no branch between the gathers: a lane with
+                                // one start loads row y_off twice
+                                px[i][0] = xj[rb * rs];
+                                px[i][1] = xj[(rb + static_cast<uint32_t>(nb - 1)) * rs];
                             }
                         }
 #pragma unroll
@@ -608,6 +616,15 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                                     mid = (r + 1 == f2) ? acc : mid;
                                 }
                                 const V tot = dpp_val<kDppWaveShr1, 0xf>(acc) + red; // lanes >= 1: segment y_off - 1
+                                // x at row y_off - 1, the last start of the lane
+                                // below, taken here, after the gathers have been
+                                // waited for, not between them (FP64 k = 8 with
+                                // the dot: 580 us, against 695 us with the shift
+                                // right after the loads and 618 us with three
+                                // gathers per lane)
+                                V pa = V(0);
+                                if constexpr (DOT)
+                                    pa = dpp_val<kDppWaveShr1, 0xf>(px[i][1]);
                                 V* yj = y + static_cast<size_t>(j) * ycs;
                                 auto store_row = [&](int seg, V val, V xr) {
                                     yj[(yrow0 + seg) * rs] = val;
@@ -615,14 +632,14 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                                         dacc[j] += static_cast<double>(xr) * static_cast<double>(val);
                                 };
                                 if (lane >= 1 && (keep0 || lane != 1 || two0))
-                                    store_row(y_off - 1, tot, px[i][0]);
+                                    store_row(y_off - 1, tot, pa);
                                 if (nb == 2 && (keep0 || lane != 0))
-                                    store_row(y_off, mid, px[i][1]);
+                                    store_row(y_off, mid, px[i][0]);
                                 if (!keep0) {
                                     const V s0 = carry[j] + (two0 ? lane_value(mid, 0) : lane_value(tot, 1));
                                     if (lane == 0) {
                                         if (!carry_is_head)
-                                            store_row(0, s0, px[i][1]);
+                                            store_row(0, s0, px[i][0]);
                                         else if (!drop_head)
                                             static_cast<V*>(p.head)[static_cast<size_t>(c) * k + j] = s0;
                                     }
@@ -631,7 +648,7 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                                 const V tv = lane_value(acc, 63);
                                 if (complete) {
                                     if (lane == 63)
-                                        store_row(nseg - 1, tv, nb == 2 ? px[i][2] : px[i][1]);
+                                        store_row(nseg - 1, tv, px[i][1]);
                                     carry[j] = V(0);
                                 } else {
                                     carry[j] = tv;
@@ -762,7 +779,7 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                 } else {
 #pragma unroll 1
                     for (int r = 0; r < SIGMA; ++r)
-                        step(r, ldg_stream<!DOT>(vals + r * 64), ldg_stream<!DOT>(cols + r * 64));
+                        step(r, ldg_stream<NT>(vals + r * 64), ldg_stream<NT>(cols + r * 64));
                 }
 
                 V blue[GG];
@@ -999,7 +1016,7 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
     return 0.0;
 }
 
-template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES = false>
+template<typename V, int SIGMA, int KMAX, bool DOT, bool CODES = false, bool NT = !DOT>
 __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
     if (p.done && *p.done)
         return;
@@ -1031,7 +1048,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
             i = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
         }
         while (i < n) { // wave-uniform
-            (void)csr5_chunk<V, SIGMA, KMAX, DOT, CODES>(p, c0 + i, false);
+            (void)csr5_chunk<V, SIGMA, KMAX, DOT, CODES, NT>(p, c0 + i, false);
             if (!sweep)
                 break;
             i = __builtin_amdgcn_readfirstlane(atomicAdd(q, lane == 0 ? 1u : 0u));
@@ -1057,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_csr5_tiles(TileParams p) {
         if (p.chunk_sys && p.sys_done)
             stopped = sload(p.sys_done + sload(p.chunk_sys + c)) != 0;
     }
-    const double s = csr5_chunk<V, SIGMA, KMAX, DOT, CODES>(p, c, stopped);
+    const double s = csr5_chunk<V, SIGMA, KMAX, DOT, CODES, NT>(p, c, stopped);
     if constexpr (DOT && KMAX == 1) {
         const int lane = threadIdx.x & 63;
         if (p.chunk_tile) {
@@ -1282,23 +1299,42 @@ Plan plan_by_sigma(uint32_t sigma, uint32_t ntiles, bool codes) {
     }
 }
 
+// nt: single-vector products with the dot stream the matrix non-temporally
+// (the other products' choice is fixed by their template arguments).
+template<typename V, int S, int KMAX, bool DOT, bool CODES>
+void launch_tiles(const TileParams& tp, uint32_t grid, hipStream_t st, bool nt) {
+    if constexpr (KMAX == 1 && DOT) {
+        if (nt) {
+            hipLaunchKernelGGL((k_csr5_tiles<V, S, KMAX, DOT, CODES, true>), dim3(grid), dim3(256), 0, st, tp);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_csr5_tiles<V, S, KMAX, DOT, CODES>), dim3(grid), dim3(256), 0, st, tp);
+}
+
 template<typename V, int KMAX, bool DOT>
-void launch_tiles_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, hipStream_t st, bool codes) {
+void launch_tiles_by_sigma(uint32_t sigma, const TileParams& tp, uint32_t grid, hipStream_t st, bool codes,
+                           bool nt) {
     switch (sigma) {
 #define DIM_TILES_CASE(S)                                                                                             \
     case S:                                                                                                           \
         if constexpr (coded_kernel<KMAX, S>())                                                                        \
             if (codes) {                                                                                              \
-                hipLaunchKernelGGL((k_csr5_tiles<V, S, KMAX, DOT, true>), dim3(grid), dim3(256), 0, st, tp);          \
+                launch_tiles<V, S, KMAX, DOT, true>(tp, grid, st, nt);                                                \
                 break;                                                                                                \
             }                                                                                                         \
-        hipLaunchKernelGGL((k_csr5_tiles<V, S, KMAX, DOT, false>), dim3(grid), dim3(256), 0, st, tp);                 \
+        launch_tiles<V, S, KMAX, DOT, false>(tp, grid, st, nt);                                                       \
         break;
         DIM_FOR_EACH_SIGMA(DIM_TILES_CASE)
 #undef DIM_TILES_CASE
         default: break;
     }
 }
+
+// Whether a solver's product (fused with the dot) should stream the matrix
+// non-temporally: its working set, the matrix streams and the solver's four
+// vectors (x, r, p, q) of vb-byte values, exceeds the 256 MB Infinity Cache.
+bool solve_streams_matrix(const DevFormat& A, size_t vb);
 
 // Launches the tile kernel and the fix-up kernel; records `mid` between them
 // when given. Returns the number of dot partials per column and, in *parts,
@@ -1352,9 +1388,10 @@ uint32_t launch_pair(const DevFormat& A, TileParams tp, FixParams fp, bool dot, 
     }
     if (pl.tgrid) {
         if (dot)
-            launch_tiles_by_sigma<V, KMAX, true>(A.sigma, tp, pl.tgrid, st, codes);
+            launch_tiles_by_sigma<V, KMAX, true>(A.sigma, tp, pl.tgrid, st, codes,
+                                                 KMAX == 1 && solve_streams_matrix(A, sizeof(V)));
         else
-            launch_tiles_by_sigma<V, KMAX, false>(A.sigma, tp, pl.tgrid, st, codes);
+            launch_tiles_by_sigma<V, KMAX, false>(A.sigma, tp, pl.tgrid, st, codes, false);
     }
     if (mid)
         (void)hipEventRecord(mid, st);

@@ -75,6 +75,28 @@ def test_batched_cg_more_blocks_than_resident(cuda):
         off += n
 
 
+def test_batched_cg_beyond_infinity_cache_matches_single_solves(cuda):
+    """Four systems of 8.0M rows in all (40M stored elements): the matrix and
+    the solver's four vectors take about 0.6 GB, more than the 256 MB
+    Infinity Cache, so the batch's products stream the matrix with
+    non-temporal loads (the single solves of each system fit and use
+    ordinary loads). Each system matches its own single-system solve."""
+    systems = _systems([1500, 1300, 1700, 1100], 0.5, seed=21)
+    A = Csr5.from_csr_batched(systems, sigma=5)
+    rng = np.random.default_rng(8)
+    bs = [rng.uniform(-1, 1, n) for n, *_ in systems]
+    X, res = conjugate_gradient_batched(A, to_dev(np.concatenate(bs)), rtol=1e-10, max_iters=30000)
+    X = X.cpu().numpy()
+    off = 0
+    for (n, rp, ci, v), b, r in zip(systems, bs, res):
+        M = Csr5.from_csr((n, n, rp, ci, v), sigma=5)
+        xs, rs = conjugate_gradient(M, to_dev(b), rtol=1e-10, max_iters=30000)
+        assert r.converged and _iters_close(r.iterations, rs.iterations), (r.iterations, rs.iterations)
+        assert r.relres_true <= 1.01e-10, r.relres_true
+        np.testing.assert_allclose(X[off: off + n], xs.cpu().numpy(), rtol=1e-7, atol=1e-8)
+        off += n
+
+
 def test_batched_products_and_values(cuda):
     import torch
 

@@ -54,6 +54,26 @@ def test_cg_poisson3d_128_full_size(cuda, sigma):
     assert err < 1e-6
 
 
+@pytest.mark.parametrize("sigma", [7, 16])
+def test_cg_beyond_infinity_cache_matches_oracle(cuda, sigma):
+    """168^3 Poisson (4.7M rows, 33M stored elements): the matrix and the
+    solver's four vectors take 0.44 GB (sigma 7, column codes) or 0.56 GB
+    (sigma 16), more than the 256 MB Infinity Cache, so the product fused
+    with p.q streams the matrix with non-temporal loads. Same iteration
+    count as the oracle's CG within +-2, and the same solution."""
+    rp, ci, v = matgen.poisson3d(168)
+    n = 168 ** 3
+    xs, b = _problem(rp, ci, v, n, 168)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=sigma)
+    x, res = conjugate_gradient(A, to_dev(b), rtol=1e-8, max_iters=5000)
+    oc = O.Csr5(n, n, rp, ci, v).cg(b, rtol=1e-8, maxit=5000, nthreads=16)
+    assert res.converged and oc["converged"]
+    assert abs(res.iterations - oc["iterations"]) <= 2, (res.iterations, oc["iterations"])
+    assert res.relres_true <= 2e-8
+    err = np.linalg.norm(x.cpu().numpy() - oc["x"]) / np.linalg.norm(oc["x"])
+    assert err < 1e-6
+
+
 @pytest.mark.parametrize("n", [73, 87])
 def test_cg_coded_products_of_short_chunks(cuda, n):
     """CG's product with the fused p.q on grids whose chunks hold two and
