@@ -294,6 +294,10 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
         carry[j] = V(0);
         dacc[j] = 0.0;
     }
+    // Coded single-vector products with the dot (DIAGX below): x at the
+    // diagonal element of the carried row, when an earlier tile held it.
+    V carry_dx = V(0);
+    bool carry_dx_ok = false;
     const uint32_t tf0 = sload_flag(p.tile_flags + t0);
     const bool chunk_starts = tf0 & kStartsRow;
     bool carry_is_head = !chunk_starts;
@@ -339,6 +343,19 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
     }
     constexpr bool PIPE = XREG && SIGMA <= 8;
     constexpr bool CODED = CODES && REG && SIGMA <= 8;
+    // The dot needs x at every row the product writes. A coded single-vector
+    // product takes it from its own gathers where it can: the element on the
+    // diagonal (code naming offset 0) gathered x at its row. Measured on the
+    // first product of a CG solve: loading x at the rows instead costs 42.6
+    // against 38.7 us (128^3 Poisson), 81.7 against 73.9 us (32 uneven 2-D
+    // systems). Keeping those x values takes 9 more VGPRs, which at tile
+    // heights 4 and 7 costs a wavefront per SIMD (90 -> 99 and 125 -> 134
+    // VGPRs); at 7 that made the 128^3 Poisson CG slower (20.8 -> 21.9 ms),
+    // so only tile heights 5 and 6 (4 wavefronts per SIMD either way) take
+    // it: 32 uneven 2-D systems at tile height 5, 0.603 -> 0.587 s per solve
+    // (0.590 s with the diagonal mask formed after the gathers instead of
+    // with the decode).
+    constexpr bool DIAGX = DOT && KMAX == 1 && CODED && (SIGMA == 5 || SIGMA == 6);
     constexpr int NREG = REG ? SIGMA : 1;
     constexpr int NX = XREG ? SIGMA : 1;
     const V* __restrict__ vals_all = static_cast<const V*>(p.vals);
@@ -352,6 +369,7 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
         V v[NREG];
         uint32_t col[NREG];
         uint32_t code;
+        uint32_t dg; // DIAGX: steps whose element lies on the diagonal
         W bits;
         uint32_t tile_ptr, tdo_at;
         uint32_t tf, tf_next;
@@ -408,6 +426,15 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                 const uint32_t row = rb + static_cast<uint32_t>(__popc(bits & ((2u << r) - 1u)));
                 s.col[r] = min(row + static_cast<uint32_t>(d[r]), p.cols - 1u);
             }
+            // (with the decode, while the tile's gathers are in flight)
+            if constexpr (DIAGX) {
+                uint32_t dg = 0;
+#pragma unroll
+                for (int r = 0; r < SIGMA; ++r)
+                    dg |= (d[r] == 0 ? 1u : 0u) << r;
+                s.dg = dg;
+            }
+
         }
     };
     // Up to sigma 7 the single-vector pipeline runs two tiles ahead (two
@@ -580,6 +607,36 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                 // start of the lane below (every lane holds one or two
                 // starts here), from that lane.
                 const size_t rb = row0 + static_cast<uint32_t>(y_off);
+                // DIAGX: the lane's diagonal elements among its head (steps
+                // before its first start: the row of the lane below), its first
+                // row (complete in the lane when it holds two starts) and its
+                // last row. The row a lane closes for the lane below has its
+                // diagonal in the head or in the lane below's tail; segment 0
+                // of a tile that continues a row, in an earlier tile, in lane
+                // 0, or (lane 0 holding that row alone) in lane 1's head; the
+                // tile's last row, in lane 63's tail. A wave where some written
+                // row has none of these (a row without a stored diagonal)
+                // loads x at the rows instead.
+                uint32_t hm = 0, fm = 0, tm = 0;
+                bool dfall = false;
+                int fv0 = 0, hv1 = 0;
+                if constexpr (DIAGX) {
+                    const uint32_t dg = cu.dg;
+                    const uint32_t below1 = (1u << f1) - 1u;
+                    hm = dg & below1;
+                    fm = dg & ~below1 & (nb == 2 ? ((1u << f2) - 1u) : ~0u);
+                    tm = dg & ~((1u << (nb == 2 ? f2 : f1)) - 1u);
+                    const int tv_below = dpp_int<kDppWaveShr1, 0xf>(tm != 0 ? 1 : 0);
+                    fv0 = lane_value(fm != 0 ? 1 : 0, 0);
+                    hv1 = lane_value(hm != 0 ? 1 : 0, 1);
+                    const bool need = (lane >= 1 && (keep0 || lane != 1 || two0) && hm == 0 && !tv_below) ||
+                                      (nb == 2 && (keep0 || lane != 0) && fm == 0) ||
+                                      (lane == 0 && !keep0 && !carry_is_head && !carry_dx_ok && !fv0 &&
+                                       (two0 || !hv1)) ||
+                                      (lane == 63 && complete && tm == 0);
+                    dfall = __ballot(need) != 0ull;
+                }
+                V td_last = V(0); // DIAGX: x at the diagonal of the lane's last row (side 0)
                 constexpr int G = KMAX < smm_group<V>() ? KMAX : smm_group<V>();
 #pragma unroll
                 for (int g = 0; g < KMAX; g += G) {
@@ -597,7 +654,7 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                                 else
                                     xv[i][r] = xj[static_cast<size_t>(cu.col[r]) * rs];
                             }
-                            if constexpr (DOT) {
+                            if constexpr (DOT && !DIAGX) {
                                 // no branch between the gathers: a lane with
                                 // one start loads row y_off twice
                                 px[i][0] = xj[rb * rs];
@@ -616,15 +673,42 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                                     mid = (r + 1 == f2) ? acc : mid;
                                 }
                                 const V tot = dpp_val<kDppWaveShr1, 0xf>(acc) + red; // lanes >= 1: segment y_off - 1
-                                // x at row y_off - 1, the last start of the lane
-                                // below, taken here, after the gathers have been
-                                // waited for, not between them (FP64 k = 8 with
-                                // the dot: 580 us, against 695 us with the shift
-                                // right after the loads and 618 us with three
-                                // gathers per lane)
-                                V pa = V(0);
-                                if constexpr (DOT)
+                                // x at the rows this lane writes: row y_off - 1
+                                // (pa), its first start's row (pf), its last
+                                // start's row (pl) and, in lane 0, segment 0's
+                                // row (ps0). Without DIAGX, x at row y_off - 1
+                                // is the lane below's last-start load, shifted
+                                // here, after the gathers have been waited for,
+                                // not between them (FP64 k = 8 with the dot:
+                                // 580 us, against 695 us with the shift right
+                                // after the loads and 618 us with three
+                                // gathers per lane).
+                                V pa = V(0), pf = V(0), pl = V(0), ps0 = V(0);
+                                if constexpr (DIAGX) {
+                                    V hd = V(0), fd = V(0), td = V(0); // x at the diagonal elements
+#pragma unroll
+                                    for (int r = 0; r < SIGMA; ++r) {
+                                        hd = ((hm >> r) & 1u) ? xv[i][r] : hd;
+                                        fd = ((fm >> r) & 1u) ? xv[i][r] : fd;
+                                        td = ((tm >> r) & 1u) ? xv[i][r] : td;
+                                    }
+                                    const V td_below = dpp_val<kDppWaveShr1, 0xf>(td);
+                                    pa = hm ? hd : td_below;
+                                    pf = fd;
+                                    pl = td;
+                                    const V fd0 = lane_value(fd, 0), hd1 = lane_value(hd, 1);
+                                    ps0 = carry_dx_ok ? carry_dx : (fv0 ? fd0 : hd1);
+                                    td_last = td;
+                                    if (dfall) { // wave-uniform
+                                        pf = ps0 = x[rb];
+                                        pl = x[rb + static_cast<uint32_t>(nb - 1)];
+                                        pa = dpp_val<kDppWaveShr1, 0xf>(pl);
+                                    }
+                                } else if constexpr (DOT) {
                                     pa = dpp_val<kDppWaveShr1, 0xf>(px[i][1]);
+                                    pf = ps0 = px[i][0];
+                                    pl = px[i][1];
+                                }
                                 V* yj = y + static_cast<size_t>(j) * ycs;
                                 auto store_row = [&](int seg, V val, V xr) {
                                     yj[(yrow0 + seg) * rs] = val;
@@ -634,12 +718,12 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                                 if (lane >= 1 && (keep0 || lane != 1 || two0))
                                     store_row(y_off - 1, tot, pa);
                                 if (nb == 2 && (keep0 || lane != 0))
-                                    store_row(y_off, mid, px[i][0]);
+                                    store_row(y_off, mid, pf);
                                 if (!keep0) {
                                     const V s0 = carry[j] + (two0 ? lane_value(mid, 0) : lane_value(tot, 1));
                                     if (lane == 0) {
                                         if (!carry_is_head)
-                                            store_row(0, s0, px[i][0]);
+                                            store_row(0, s0, ps0);
                                         else if (!drop_head)
                                             static_cast<V*>(p.head)[static_cast<size_t>(c) * k + j] = s0;
                                     }
@@ -648,7 +732,7 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                                 const V tv = lane_value(acc, 63);
                                 if (complete) {
                                     if (lane == 63)
-                                        store_row(nseg - 1, tv, px[i][1]);
+                                        store_row(nseg - 1, tv, pl);
                                     carry[j] = V(0);
                                 } else {
                                     carry[j] = tv;
@@ -659,6 +743,11 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
                 }
                 if (!keep0)
                     carry_is_head = false;
+                if constexpr (DIAGX) {
+                    // the row carried into the next tile started in lane 63
+                    carry_dx_ok = !complete && lane_value(tm != 0 ? 1 : 0, 63) != 0;
+                    carry_dx = lane_value(td_last, 63);
+                }
             }
         } else {
         const int incl = wave_incl_scan(nb, lane);
@@ -902,6 +991,8 @@ __device__ __forceinline__ double csr5_chunk(const TileParams& p, const uint32_t
             process(std::true_type{});
         else
             process(std::false_type{});
+        if constexpr (DIAGX)
+            carry_dx_ok = false; // this path loads x at the rows it writes
         }
 
     };

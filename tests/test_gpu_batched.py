@@ -423,7 +423,7 @@ def test_batched_rows_crossing_chunks(cuda, sigma, persistent):
         off += n
 
 
-@pytest.mark.parametrize("sigma", [4, 5, 7, 8])
+@pytest.mark.parametrize("sigma", [4, 5, 6, 7, 8])
 def test_batched_column_codes_per_system(cuda, sigma):
     """Stencil systems of different sides keep column codes, each with its own
     table of diagonals (col - row = -s, -1, 0, 1, s): the batch's products,

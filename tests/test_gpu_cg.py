@@ -17,7 +17,7 @@ def _problem(rp, ci, v, n, seed):
     return xs, b
 
 
-@pytest.mark.parametrize("sigma", [8, 16, 32])
+@pytest.mark.parametrize("sigma", [5, 6, 8, 16, 32])
 def test_cg_poisson2d_256_matches_oracle(cuda, sigma):
     """2-D 5-point Laplacian 256^2 (the CLI workload), rtol 1e-10."""
     rp, ci, v = matgen.poisson2d(256)
@@ -72,6 +72,58 @@ def test_cg_beyond_infinity_cache_matches_oracle(cuda, sigma):
     assert res.relres_true <= 2e-8
     err = np.linalg.norm(x.cpu().numpy() - oc["x"]) / np.linalg.norm(oc["x"])
     assert err < 1e-6
+
+
+def _banded_blocks(n, block, width, seed):
+    """Symmetric, diagonally dominant matrix whose rows alternate, block by
+    block, between a band of half-width `width` and a tridiagonal band:
+    2 * width + 1 diagonals in all (column codes), wide rows that leave lanes
+    without a row start (the general path) next to short rows (the fast
+    path)."""
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        w = width if (i // block) % 2 == 0 else 1
+        for d in range(1, w + 1):
+            j = i + d
+            if j < n:
+                a = -rng.uniform(0.1, 1.0)
+                rows += [i, j]
+                cols += [j, i]
+                vals += [a, a]
+    rows = np.array(rows, dtype=np.int64)
+    cols = np.array(cols, dtype=np.int64)
+    vals = np.array(vals)
+    diag = 1.0 + np.bincount(rows, weights=np.abs(vals), minlength=n)
+    rows = np.concatenate([rows, np.arange(n)])
+    cols = np.concatenate([cols, np.arange(n)])
+    vals = np.concatenate([vals, diag])
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    rp = np.zeros(n + 1, dtype=np.uint32)
+    rp[1:] = np.cumsum(np.bincount(rows, minlength=n))
+    return rp, cols.astype(np.uint32), vals
+
+
+@pytest.mark.parametrize("sigma", [5, 6])
+def test_cg_coded_wide_and_short_rows(cuda, sigma):
+    """Column-coded CG product with the fused dot where tiles of wide rows
+    (general path) alternate with tiles of short rows (fast path, x at the
+    rows taken from the diagonal elements' gathers): a row carried out of a
+    general-path tile, or with its diagonal in an earlier tile, falls back to
+    loading x at the rows. Iteration count within +-2 of the oracle's CG and
+    the same solution."""
+    n = 24000
+    rp, ci, v = _banded_blocks(n, 700, 7, sigma)
+    xs, b = _problem(rp, ci, v, n, 11)
+    A = Csr5.from_csr((n, n, rp, ci, v), sigma=sigma)
+    assert A.info.column_codes == 1
+    x, res = conjugate_gradient(A, to_dev(b), rtol=1e-10, max_iters=5000)
+    oc = O.Csr5(n, n, rp, ci, v).cg(b, rtol=1e-10, maxit=5000)
+    assert res.converged and oc["converged"]
+    assert abs(res.iterations - oc["iterations"]) <= 2, (res.iterations, oc["iterations"])
+    assert res.relres_true <= 1.01e-10
+    np.testing.assert_allclose(x.cpu().numpy(), oc["x"], rtol=1e-8, atol=1e-10)
 
 
 @pytest.mark.parametrize("n", [73, 87])
