@@ -36,6 +36,7 @@ void free_handle(dim_csr5* h) {
     int prev = 0;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(h->device);
+    free_cg_cache(h->cg_cache);
     for (void* p : h->allocations)
         (void)hipFree(p);
     (void)hipSetDevice(prev);

@@ -32,6 +32,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -397,12 +398,15 @@ struct DevBuf {
 struct Graph {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
-    ~Graph() {
+    void reset() {
         if (ge)
             (void)hipGraphExecDestroy(ge);
         if (g)
             (void)hipGraphDestroy(g);
+        ge = nullptr;
+        g = nullptr;
     }
+    ~Graph() { reset(); }
 };
 
 struct PinnedState {
@@ -473,6 +477,79 @@ dim_status alloc_work(DevBuf& buf, size_t n, int k, uint32_t nvb, bool mixed, in
     return DIM_SUCCESS;
 }
 
+// Bytes of the work buffers alloc_work() allocates.
+size_t work_bytes(size_t n, int k, uint32_t nvb, bool mixed, int hist_cap) {
+    const size_t nk = n * static_cast<size_t>(k);
+    return sizeof(CgState) + 4 * size_t{nvb} * kMaxK * sizeof(double) + static_cast<size_t>(hist_cap) * k * 8 +
+           nk * 8 /* q64 */ + nk * 16 /* r64, p64 or r32, p32, q32, x32 */ + (k > 1 ? nk * 16 : 0);
+}
+
+} // namespace
+
+// A handle's last solve: its work buffers, the pinned copy of the state, the
+// capture stream and the captured batch of iterations, reused by the next
+// solve on the handle with the same rows, right-hand sides, precision and
+// history length (and, for the graph, the same iteration kernels, batch
+// length and x). A small solve then spends its time in the iterations, not
+// in hipMalloc, hipHostMalloc and graph instantiation. Solves whose buffers
+// exceed kCgCacheMaxBytes allocate per call instead, so that a handle never
+// holds more than that between solves.
+struct CgCache {
+    DevBuf buf;
+    Work w;
+    size_t n = 0;
+    int k = 0;
+    bool mixed = false;
+    int hist_cap = 0;
+    PinnedState pinned;
+    hipStream_t cap = nullptr;
+    Graph graph;
+    int g_kmax = 0;
+    uint32_t g_check = 0;
+    const void* g_x = nullptr;
+    ~CgCache() {
+        graph.reset();
+        if (cap)
+            (void)hipStreamDestroy(cap);
+    }
+};
+
+void free_cg_cache(CgCache* c) {
+    delete c;
+}
+
+namespace {
+
+constexpr size_t kCgCacheMaxBytes = size_t{256} << 20;
+
+// The handle's cache for a solve of this shape, made (and the old one freed)
+// when the shape changed; nullptr when the buffers are larger than the cache
+// keeps.
+dim_status cg_cache_for(dim_csr5* A, size_t n, int k, uint32_t nvb, bool mixed, int hist_cap, CgCache** out) {
+    *out = nullptr;
+    if (work_bytes(n, k, nvb, mixed, hist_cap) > kCgCacheMaxBytes)
+        return DIM_SUCCESS;
+    CgCache* c = A->cg_cache;
+    if (c && c->n == n && c->k == k && c->mixed == mixed && c->hist_cap == hist_cap) {
+        *out = c;
+        return DIM_SUCCESS;
+    }
+    free_cg_cache(c);
+    A->cg_cache = nullptr;
+    std::unique_ptr<CgCache> fresh(new CgCache);
+    if (dim_status e = alloc_work(fresh->buf, n, k, nvb, mixed, hist_cap, fresh->w))
+        return e;
+    DIM_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&fresh->pinned.h), sizeof(CgState), hipHostMallocDefault));
+    DIM_HIP_TRY(hipStreamCreateWithFlags(&fresh->cap, hipStreamNonBlocking));
+    fresh->n = n;
+    fresh->k = k;
+    fresh->mixed = mixed;
+    fresh->hist_cap = hist_cap;
+    A->cg_cache = fresh.release();
+    *out = A->cg_cache;
+    return DIM_SUCCESS;
+}
+
 // fixed == nullptr: the solve allocates its buffers, captures a batch of
 // iterations in a hipGraph and polls a pinned copy of the state. Otherwise its
 // buffers are the ones `fixed` points into and it allocates nothing: no
@@ -494,10 +571,16 @@ dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const di
 
     DevBuf buf;
     Work w;
+    CgCache* cc = nullptr; // the handle's buffers, when this solve's fit them
     if (no_alloc) {
         w = *fixed;
-    } else if (dim_status e = alloc_work(buf, n, k, nvb, mixed, hist_cap, w)) {
-        return e;
+    } else {
+        if (dim_status e = cg_cache_for(A, n, k, nvb, mixed, hist_cap, &cc))
+            return e;
+        if (cc)
+            w = cc->w;
+        else if (dim_status e = alloc_work(buf, n, k, nvb, mixed, hist_cap, w))
+            return e;
     }
     CgState* st = w.st;
     double *q64 = w.q64, *r64 = w.r64, *p64 = w.p64, *rr0 = w.rr0, *rr1 = w.rr1, *bb = w.bb, *rt = w.rt,
@@ -508,7 +591,9 @@ dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const di
     PinnedState pinned;
     CgState stack_state{};
     CgState* hs = &stack_state;
-    if (!no_alloc) {
+    if (cc) {
+        hs = cc->pinned.h;
+    } else if (!no_alloc) {
         DIM_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&pinned.h), sizeof(CgState), hipHostMallocDefault));
         hs = pinned.h;
     }
@@ -528,21 +613,39 @@ dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const di
             }
         }
     };
-    // capture `check` iterations (even) once
+    // capture `check` iterations (even) once, or reuse the handle's capture
+    // of the same iterations
     Graph graph;
-    bool use_graph = false;
-    if (!no_alloc) {
-        hipStream_t cap = nullptr;
-        DIM_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-        use_graph = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
-        if (use_graph) {
+    hipGraphExec_t exec = nullptr;
+    if (cc && cc->graph.ge && cc->g_kmax == KMAX && cc->g_check == check && cc->g_x == x) {
+        exec = cc->graph.ge;
+    } else if (!no_alloc) {
+        Graph& gr = cc ? cc->graph : graph;
+        gr.reset();
+        hipStream_t cap = cc ? cc->cap : nullptr;
+        if (!cc)
+            DIM_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
             enqueue_batch(cap);
-            use_graph = hipStreamEndCapture(cap, &graph.g) == hipSuccess && graph.g &&
-                        hipGraphInstantiate(&graph.ge, graph.g, nullptr, nullptr, 0) == hipSuccess;
+            ok = hipStreamEndCapture(cap, &gr.g) == hipSuccess && gr.g &&
+                 hipGraphInstantiate(&gr.ge, gr.g, nullptr, nullptr, 0) == hipSuccess;
             (void)hipGetLastError();
         }
-        (void)hipStreamDestroy(cap);
+        if (!cc)
+            (void)hipStreamDestroy(cap);
+        if (ok) {
+            exec = gr.ge;
+            if (cc) {
+                cc->g_kmax = KMAX;
+                cc->g_check = check;
+                cc->g_x = x;
+            }
+        } else {
+            gr.reset();
+        }
     }
+    const bool use_graph = exec != nullptr;
 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const auto t_start = std::chrono::steady_clock::now();
@@ -616,7 +719,7 @@ dim_status solve(dim_csr5* A, int k, const double* b_in, double* x_out, const di
         if (hs->done)
             break;
         if (use_graph) {
-            DIM_HIP_TRY(hipGraphLaunch(graph.ge, stream));
+            DIM_HIP_TRY(hipGraphLaunch(exec, stream));
         } else {
             enqueue_batch(stream);
             DIM_HIP_TRY(hipGetLastError());

@@ -11,6 +11,11 @@
 #include <string>
 #include <vector>
 
+namespace dimgpu {
+struct CgCache;
+void free_cg_cache(CgCache* c);
+} // namespace dimgpu
+
 struct dim_csr5 {
     dimgpu::DevFormat f;
     dimgpu::Scratch s;
@@ -47,6 +52,10 @@ struct dim_csr5 {
     // (dim_csr5_spmv_host, dim_cg_host): cols + rows FP64-sized entries,
     // allocated on first use
     void* host_stage = nullptr;
+    // work buffers, pinned state and captured iteration graph of the last
+    // dim_cg / dim_cg_multi solve, kept for the next solve of the same shape
+    // (cg.hip); freed with the handle
+    dimgpu::CgCache* cg_cache = nullptr;
 };
 
 namespace dimgpu {

@@ -204,7 +204,10 @@ typedef struct dim_cg_result {
 /* conjugate_gradient (thesis.tex:2133-2160; step order of
  * benchmarks/conjugate_gradient/dim_sp/main.cpp:102-127): solves A x = b for
  * SPD A. b and x are DEVICE FP64 arrays of rows entries; x holds x0 on
- * entry. Synchronises `stream` before returning. */
+ * entry. Synchronises `stream` before returning. The handle keeps the work
+ * buffers (when they take at most 256 MiB) and the captured iterations of its
+ * last dim_cg / dim_cg_multi / dim_cg_host solve for the next solve of the
+ * same shape, and frees them with the handle. */
 dim_status dim_cg(const dim_csr5* A, const double* b, double* x, const dim_cg_opts* opts, dim_cg_result* res,
                   void* stream);
 

@@ -167,6 +167,46 @@ def test_cg_host_pointers_equal_device_solve(cuda):
     np.testing.assert_array_equal(xc, xcd.cpu().numpy())
 
 
+def test_cg_repeated_solves_on_one_handle_equal_fresh_handles(cuda):
+    """A handle keeps its last solve's work buffers and captured iterations
+    for the next solve of the same shape. Solves in a row on one handle (the
+    same x buffer again, another x buffer, another b, other right-hand-side
+    counts, precisions and history lengths, then back) equal the same solves
+    on fresh handles bit for bit."""
+    import torch
+
+    rp, ci, v = matgen.varcoef((40, 40), 0.5, seed=7)
+    n = 1600
+    csr = (n, n, rp, ci, v)
+    A = Csr5.from_csr(csr, sigma=5, keep_f32=True)
+    rng = np.random.default_rng(4)
+    b1, b2 = to_dev(rng.uniform(-1, 1, n)), to_dev(rng.uniform(-1, 1, n))
+    B3 = to_dev(rng.uniform(-1, 1, (n, 3)))
+    xbuf = torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    def single(M, b, x=None, **kw):
+        xs, r = conjugate_gradient(M, b, None if x is None else x.zero_(), rtol=1e-10, max_iters=3000, **kw)
+        return xs.clone(), r
+
+    steps = [
+        lambda M: single(M, b1),
+        lambda M: single(M, b2, xbuf),
+        lambda M: single(M, b1, xbuf),                       # same x buffer: the captured iterations again
+        lambda M: single(M, b1, history=True),
+        lambda M: single(M, b2, precision="f32-refined"),
+        lambda M: conjugate_gradient_multi(M, B3, rtol=1e-10, max_iters=3000),
+        lambda M: single(M, b2, xbuf),
+    ]
+    for step in steps:
+        x_kept, r_kept = step(A)
+        x_new, r_new = step(Csr5.from_csr(csr, sigma=5, keep_f32=True))
+        assert torch.equal(x_kept, x_new)
+        rk = r_kept if isinstance(r_kept, list) else [r_kept]
+        rn = r_new if isinstance(r_new, list) else [r_new]
+        assert [r.iterations for r in rk] == [r.iterations for r in rn]
+        assert all(r.converged for r in rk)
+
+
 def test_cg_is_bitwise_reproducible(cuda):
     rp, ci, v = matgen.poisson3d(40)
     n = 40 ** 3
