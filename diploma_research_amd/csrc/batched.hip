@@ -911,39 +911,77 @@ dim_status report(dim_csr5* A, const double* b, double* x, bool transpose, const
     return status;
 }
 
-dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_opts& o, dim_cg_result* res,
-                         hipStream_t stream) {
+} // namespace
+
+// The buffers of a batched solve, all sized by the batch alone: the block
+// tables, the state, the per-system arrays, q, r, p and the partials, the
+// pinned copy of the state, the capture stream and the captured batch of
+// iterations. A handle keeps them (batch_cache) from its first solve when
+// q, r and p take at most kBatchCacheMaxBytes; the graph is reused while
+// the direction (A or A^T), the batch length and x stay the same. Every
+// buffer is written before it is read.
+struct BatchCache {
+    DevMem m;
+    std::vector<uint32_t> blk_sys, blk_r0, blk_r1, sys_blk;
+    uint32_t B = 0;
+    Tables tb{};
+    uint32_t* d_sb = nullptr;
+    BState* st = nullptr;
+    SysArrays sa{};
+    double *q = nullptr, *r = nullptr, *p = nullptr, *rr = nullptr, *pq = nullptr, *bb = nullptr, *rt = nullptr;
+    BState* hs = nullptr; // pinned
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool g_transpose = false;
+    uint32_t g_check = 0;
+    const double* g_x = nullptr;
+    void reset_graph() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~BatchCache() {
+        reset_graph();
+        if (cap)
+            (void)hipStreamDestroy(cap);
+        if (hs)
+            (void)hipHostFree(hs);
+    }
+};
+
+void free_batch_cache(BatchCache* c) {
+    delete c;
+}
+
+namespace {
+
+constexpr size_t kBatchCacheMaxBytes = size_t{256} << 20;
+
+dim_status init_batch_cache(dim_csr5* A, BatchCache& c) {
     const uint32_t S = A->n_systems;
-    const bool transpose = (o.flags & DIM_CG_TRANSPOSE) != 0;
-    const bool codes = A->f.col_code != nullptr;
-    const bool persistent = (o.flags & DIM_CG_PERSISTENT) != 0;
-    const int occ = persistent ? persistent_occupancy(A->f.sigma, codes) : 0;
-    if (persistent && occ < 1)
-        return fail(DIM_UNSUPPORTED, "batched CG: the persistent kernel cannot be resident for this format");
+    const size_t n = A->f.rows;
     // block tables: one or more blocks of kRowsPerBlock rows per system, none
     // shared (the vector kernels, the residuals and the persistent kernel's
     // work items)
-    std::vector<uint32_t> blk_sys, blk_r0, blk_r1, sys_blk;
-    make_items(A, kRowsPerBlock, blk_sys, blk_r0, blk_r1, sys_blk);
-    const uint32_t B = static_cast<uint32_t>(blk_sys.size());
-    const size_t n = A->f.rows;
-    const uint32_t check = o.check_every ? o.check_every : 16;
-
-    DevMem m;
-    Tables tb{};
-    uint32_t *d_bs, *d_r0, *d_r1, *d_sb;
-    DIM_HIP_TRY(m.upload(d_bs, blk_sys));
-    DIM_HIP_TRY(m.upload(d_r0, blk_r0));
-    DIM_HIP_TRY(m.upload(d_r1, blk_r1));
-    DIM_HIP_TRY(m.upload(d_sb, sys_blk));
-    tb.blk_sys = d_bs;
-    tb.blk_r0 = d_r0;
-    tb.blk_r1 = d_r1;
-    tb.sys_blk = d_sb;
-    BState* st;
-    SysArrays sa{};
-    double *q, *r, *p, *rr, *pq, *bb, *rt;
-    DIM_HIP_TRY(m.alloc(st, 1));
+    make_items(A, kRowsPerBlock, c.blk_sys, c.blk_r0, c.blk_r1, c.sys_blk);
+    c.B = static_cast<uint32_t>(c.blk_sys.size());
+    const uint32_t B = c.B;
+    DevMem& m = c.m;
+    uint32_t *d_bs, *d_r0, *d_r1;
+    DIM_HIP_TRY(m.upload(d_bs, c.blk_sys));
+    DIM_HIP_TRY(m.upload(d_r0, c.blk_r0));
+    DIM_HIP_TRY(m.upload(d_r1, c.blk_r1));
+    DIM_HIP_TRY(m.upload(c.d_sb, c.sys_blk));
+    c.tb.blk_sys = d_bs;
+    c.tb.blk_r0 = d_r0;
+    c.tb.blk_r1 = d_r1;
+    c.tb.sys_blk = c.d_sb;
+    SysArrays& sa = c.sa;
+    DIM_HIP_TRY(m.alloc(c.st, 1));
     DIM_HIP_TRY(m.alloc(sa.sys_done, S));
     DIM_HIP_TRY(m.alloc(sa.sys_iters, S));
     DIM_HIP_TRY(m.alloc(sa.tol2, S));
@@ -954,17 +992,52 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
     DIM_HIP_TRY(m.alloc(sa.done_alt, S));
     DIM_HIP_TRY(m.alloc(sa.rr_alt, S));
     DIM_HIP_TRY(m.alloc(sa.broke, S));
-    DIM_HIP_TRY(m.alloc(q, n));
-    DIM_HIP_TRY(m.alloc(r, n));
-    DIM_HIP_TRY(m.alloc(p, n));
-    DIM_HIP_TRY(m.alloc(rr, B));
-    DIM_HIP_TRY(m.alloc(pq, B));
-    DIM_HIP_TRY(m.alloc(bb, B));
-    DIM_HIP_TRY(m.alloc(rt, B));
+    DIM_HIP_TRY(m.alloc(c.q, n));
+    DIM_HIP_TRY(m.alloc(c.r, n));
+    DIM_HIP_TRY(m.alloc(c.p, n));
+    DIM_HIP_TRY(m.alloc(c.rr, B));
+    DIM_HIP_TRY(m.alloc(c.pq, B));
+    DIM_HIP_TRY(m.alloc(c.bb, B));
+    DIM_HIP_TRY(m.alloc(c.rt, B));
+    DIM_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&c.hs), sizeof(BState), hipHostMallocDefault));
+    DIM_HIP_TRY(hipStreamCreateWithFlags(&c.cap, hipStreamNonBlocking));
+    return DIM_SUCCESS;
+}
 
-    BState* hs = nullptr;
-    DIM_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&hs), sizeof(BState), hipHostMallocDefault));
-    std::unique_ptr<BState, void (*)(BState*)> hs_guard(hs, [](BState* v) { (void)hipHostFree(v); });
+dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_opts& o, dim_cg_result* res,
+                         hipStream_t stream) {
+    const uint32_t S = A->n_systems;
+    const bool transpose = (o.flags & DIM_CG_TRANSPOSE) != 0;
+    const bool codes = A->f.col_code != nullptr;
+    const bool persistent = (o.flags & DIM_CG_PERSISTENT) != 0;
+    const int occ = persistent ? persistent_occupancy(A->f.sigma, codes) : 0;
+    if (persistent && occ < 1)
+        return fail(DIM_UNSUPPORTED, "batched CG: the persistent kernel cannot be resident for this format");
+    const size_t n = A->f.rows;
+    const uint32_t check = o.check_every ? o.check_every : 16;
+    // the handle's buffers (kept from its first solve), or this call's
+    BatchCache local;
+    BatchCache* bc = &local;
+    if (3 * n * sizeof(double) <= kBatchCacheMaxBytes) {
+        if (!A->batch_cache) {
+            std::unique_ptr<BatchCache> fresh(new BatchCache);
+            if (dim_status e = init_batch_cache(A, *fresh))
+                return e;
+            A->batch_cache = fresh.release();
+        }
+        bc = A->batch_cache;
+    } else if (dim_status e = init_batch_cache(A, local)) {
+        return e;
+    }
+    const uint32_t B = bc->B;
+    const Tables tb = bc->tb;
+    const std::vector<uint32_t>& sys_blk = bc->sys_blk;
+    uint32_t* d_sb = bc->d_sb;
+    BState* st = bc->st;
+    const SysArrays sa = bc->sa;
+    double *q = bc->q, *r = bc->r, *p = bc->p, *rr = bc->rr, *pq = bc->pq, *bb = bc->bb, *rt = bc->rt;
+    BState* hs = bc->hs;
+    DevMem m; // the persistent kernel's own buffers
 
     hipEvent_t e0, e1;
     DIM_HIP_TRY(hipEventCreate(&e0));
@@ -1039,11 +1112,8 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
                 return fail(DIM_HIP_ERROR, "batched CG: a grid barrier of the persistent kernel timed out");
         }
     } else {
-        // capture a batch of iterations
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        hipStream_t cap = nullptr;
-        DIM_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        // capture a batch of iterations, or reuse the handle's capture of the
+        // same iterations
         auto enqueue = [&](hipStream_t s) {
             for (uint32_t it = 0; it < check; ++it) {
                 const double* pq_parts = pq;
@@ -1064,24 +1134,28 @@ dim_status solve_batched(dim_csr5* A, const double* b, double* x, const dim_cg_o
                                    static_cast<const double*>(r), p, static_cast<const double*>(rr));
             }
         };
-        bool use_graph = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
-        if (use_graph) {
-            enqueue(cap);
-            use_graph = hipStreamEndCapture(cap, &graph) == hipSuccess && graph &&
-                        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-            (void)hipGetLastError();
-        }
-        (void)hipStreamDestroy(cap);
-        struct GraphGuard {
-            hipGraph_t g;
-            hipGraphExec_t e;
-            ~GraphGuard() {
-                if (e)
-                    (void)hipGraphExecDestroy(e);
-                if (g)
-                    (void)hipGraphDestroy(g);
+        hipGraphExec_t exec = nullptr;
+        if (bc->exec && bc->g_transpose == transpose && bc->g_check == check && bc->g_x == x) {
+            exec = bc->exec;
+        } else {
+            bc->reset_graph();
+            bool ok = hipStreamBeginCapture(bc->cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                enqueue(bc->cap);
+                ok = hipStreamEndCapture(bc->cap, &bc->graph) == hipSuccess && bc->graph &&
+                     hipGraphInstantiate(&bc->exec, bc->graph, nullptr, nullptr, 0) == hipSuccess;
+                (void)hipGetLastError();
             }
-        } guard{graph, exec};
+            if (ok) {
+                exec = bc->exec;
+                bc->g_transpose = transpose;
+                bc->g_check = check;
+                bc->g_x = x;
+            } else {
+                bc->reset_graph();
+            }
+        }
+        const bool use_graph = exec != nullptr;
         for (;;) {
             DIM_HIP_TRY(hipMemcpyAsync(hs, st, sizeof(BState), hipMemcpyDeviceToHost, stream));
             DIM_HIP_TRY(hipStreamSynchronize(stream));

@@ -37,6 +37,7 @@ void free_handle(dim_csr5* h) {
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(h->device);
     free_cg_cache(h->cg_cache);
+    free_batch_cache(h->batch_cache);
     for (void* p : h->allocations)
         (void)hipFree(p);
     (void)hipSetDevice(prev);

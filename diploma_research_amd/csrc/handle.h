@@ -14,6 +14,8 @@
 namespace dimgpu {
 struct CgCache;
 void free_cg_cache(CgCache* c);
+struct BatchCache;
+void free_batch_cache(BatchCache* c);
 } // namespace dimgpu
 
 struct dim_csr5 {
@@ -56,6 +58,9 @@ struct dim_csr5 {
     // dim_cg / dim_cg_multi solve, kept for the next solve of the same shape
     // (cg.hip); freed with the handle
     dimgpu::CgCache* cg_cache = nullptr;
+    // the same for dim_cg_batched (batched.hip): block tables and buffers
+    // sized by the batch, kept from the first batched solve
+    dimgpu::BatchCache* batch_cache = nullptr;
 };
 
 namespace dimgpu {

@@ -250,7 +250,10 @@ dim_status dim_cg_multi(const dim_csr5* A, uint32_t k, const double* B, double* 
  * entries. With opts->flags & DIM_CG_TRANSPOSE the products use A^T (the
  * adjoint solve). No reference counterpart (the reference solves one system
  * per process). Padding elements of each system's last tile are stored as
- * 0 * x[first row of the system], so x must be finite. */
+ * 0 * x[first row of the system], so x must be finite. The batch handle keeps
+ * its block tables, vectors (when they take at most 256 MiB) and captured
+ * iterations from its first dim_cg_batched solve, and frees them with the
+ * handle. */
 dim_status dim_csr5_from_csr_batched(uint32_t n_systems, const uint32_t* rows, const uint64_t* nnz,
                                      const uint32_t* const* row_ptr, const uint32_t* const* col_idx,
                                      const double* const* vals, const dim_csr5_opts* opts, dim_csr5** out);

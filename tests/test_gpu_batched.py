@@ -97,6 +97,48 @@ def test_batched_cg_beyond_infinity_cache_matches_single_solves(cuda):
         off += n
 
 
+def test_batched_repeated_solves_on_one_handle_equal_fresh_handles(cuda):
+    """A batch handle keeps its block tables, buffers and captured iterations
+    from its first solve. Solves in a row on one handle (another b, the same
+    x buffer again, another x buffer, the transposed product, the persistent
+    kernel, other batch lengths) equal the same solves on fresh handles bit
+    for bit; the transposed product adds with atomics in no fixed order, so
+    that solve matches to the solver's tolerance."""
+    import torch
+
+    systems = _systems([21, 9, 33, 14], 0.5, seed=70)
+    rng = np.random.default_rng(70)
+    N = sum(n for n, *_ in systems)
+    b1, b2 = to_dev(rng.uniform(-1, 1, N)), to_dev(rng.uniform(-1, 1, N))
+    xbuf = torch.zeros(N, dtype=torch.float64, device="cuda")
+    A = Csr5.from_csr_batched(systems, sigma=5)
+
+    def run(M, b, x=None, **kw):
+        X, res = conjugate_gradient_batched(M, b, None if x is None else x.zero_(), rtol=1e-11, max_iters=4000,
+                                            **kw)
+        return X.clone(), res
+
+    steps = [
+        (True, lambda M: run(M, b1)),
+        (True, lambda M: run(M, b2, xbuf)),
+        (True, lambda M: run(M, b1, xbuf)),          # same x buffer: the captured iterations again
+        (False, lambda M: run(M, b2, transpose=True)),
+        (True, lambda M: run(M, b1, persistent=True)),
+        (True, lambda M: run(M, b2, xbuf, check_every=5)),
+        (True, lambda M: run(M, b1, xbuf)),
+    ]
+    for exact, step in steps:
+        x_kept, r_kept = step(A)
+        x_new, r_new = step(Csr5.from_csr_batched(systems, sigma=5))
+        assert all(r.converged for r in r_kept)
+        if exact:
+            assert torch.equal(x_kept, x_new)
+            assert [r.iterations for r in r_kept] == [r.iterations for r in r_new]
+        else:
+            np.testing.assert_allclose(x_kept.cpu().numpy(), x_new.cpu().numpy(), rtol=1e-8, atol=1e-10)
+            assert all(_iters_close(a.iterations, b.iterations) for a, b in zip(r_kept, r_new))
+
+
 def test_batched_products_and_values(cuda):
     import torch
 
